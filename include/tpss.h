@@ -1,0 +1,198 @@
+/*
+ * tpss.h -- C ABI of the MI355X vertex-patch Schwarz smoother library
+ * (libtpss_amd.so).
+ *
+ * Plain C types only: device buffers are passed as `void *` / `const void *`
+ * (HIP device pointers whose element type is the level's vector precision),
+ * streams as `void *` (a hipStream_t, NULL = the null stream), host arrays as
+ * typed pointers.  Every entry point returns a tpss_status; no C++ exception
+ * crosses this boundary.  The message of the last failure on the calling
+ * thread is available from tpss_last_error().
+ *
+ * Each entry point names the reference interface it replaces
+ * (jwitte08/TPSS, file:line).
+ */
+#ifndef TPSS_H
+#define TPSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum
+{
+  TPSS_OK = 0,
+  TPSS_E_INVALID_ARG = 1, /* AssertThrow on parameters (e.g. relaxation <= 0,
+                             schwarz_preconditioner.templates.h:32) */
+  TPSS_E_NONFINITE = 2,   /* AssertIsFinite(dst.l2_norm()), :238 */
+  TPSS_E_SINGULAR = 3,    /* zero eigenvalues / indefinite mass,
+                             tensor_product_matrix.h:369-371 */
+  TPSS_E_HIP = 4,         /* a HIP runtime call failed */
+  TPSS_E_NOT_CONVERGED = 5, /* SolverControl::NoConvergence */
+  TPSS_E_UNSUPPORTED = 6  /* configuration outside this build (ExcNotImplemented) */
+} tpss_status;
+
+typedef enum { TPSS_ADDITIVE = 0, TPSS_MULTIPLICATIVE = 1 } tpss_smoother_variant;
+typedef enum { TPSS_COLOR_NONE = 0, TPSS_COLOR_TILED = 1, TPSS_COLOR_REDBLACK = 2 } tpss_coloring;
+typedef enum { TPSS_F64 = 0, TPSS_F32 = 1, TPSS_BF16 = 2 } tpss_precision;
+
+typedef struct tpss_level tpss_level;
+typedef struct tpss_smoother tpss_smoother;
+typedef struct tpss_gmg tpss_gmg;
+
+/* One multigrid level: mesh, DG Q_p space, SIPG Poisson operator.
+ * Replaces Poisson::ModelProblem mesh/DoF setup (apps/poisson_problem.h:262-306)
+ * + Laplace::DG::MF::Operator (apps/laplace_integrator.h:578-743). */
+typedef struct
+{
+  int dim;           /* 2 or 3 */
+  int degree;        /* p, 1..4 */
+  int n_cells_1d;    /* n: n^dim cells on [0,1]^dim */
+  double ip_factor;  /* EquationData::ip_factor (apps/equation_data.h:35-39) */
+  const double *vertices; /* host, (n+1)^dim x dim coordinates, x fastest;
+                             NULL = uniform Cartesian mesh */
+  int vector_precision;   /* TPSS_F64 or TPSS_F32: element type of level vectors */
+} tpss_level_desc;
+
+/* Smoother settings: SchwarzPreconditioner::AdditionalData
+ * (schwarz_preconditioner.h:495-515) + SmootherRichardson::AdditionalData
+ * (smoother_richardson.h:41-48) + the patch-solve arithmetic. */
+typedef struct
+{
+  int variant;           /* tpss_smoother_variant */
+  int coloring;          /* tpss_coloring (AVP: NONE = one color like the
+                            reference without TBB, patch_info.templates.h:450-459) */
+  double relaxation;     /* local damping omega (relaxation), > 0 */
+  int reverse;           /* reverse color order */
+  int symmetrized;       /* colors 0..C-1..0 */
+  double damping_factor; /* Richardson omega_R */
+  int n_steps;           /* Richardson steps per step() */
+  int local_precision;   /* tpss_precision of the patch solves */
+} tpss_smoother_desc;
+
+typedef struct
+{
+  double time;       /* seconds accumulated (device time, HIP events) */
+  const char *description;
+  long n_calls;
+} tpss_time_info;
+
+/* ---- library ------------------------------------------------------------ */
+const char *tpss_last_error(void);
+const char *tpss_version(void);
+
+/* ---- level / operator ---------------------------------------------------- */
+int tpss_level_create(const tpss_level_desc *desc, tpss_level **out);
+int tpss_level_destroy(tpss_level *level);
+int tpss_level_n_dofs(const tpss_level *level, int64_t *n_dofs);
+/* dst = A src; Laplace::DG::MF::Operator::vmult (apps/laplace_integrator.h:578-594) */
+int tpss_operator_vmult(tpss_level *level, void *dst, const void *src, void *stream);
+/* dst = b - A x (the residual update sadd(-1, 1, rhs) after vmult,
+ * schwarz_preconditioner.templates.h:721-725) */
+int tpss_operator_residual(tpss_level *level, void *dst, const void *b, const void *x,
+                           void *stream);
+
+/* ---- smoother / preconditioner ------------------------------------------ */
+/* SchwarzPreconditioner::initialize(subdomain_handler, op, data)
+ * (schwarz_preconditioner.templates.h:3-60) incl. SubdomainHandler::reinit
+ * (subdomain_handler.templates.h:34-154) and compute_inverses (:101-123);
+ * local solvers are set up in FP64 on the device. */
+int tpss_smoother_create(tpss_level *level, const tpss_smoother_desc *desc, tpss_smoother **out);
+/* SchwarzPreconditioner::initialize(other, data): shares the local solvers
+ * (schwarz_preconditioner.templates.h:63-86), e.g. post-smoother with reversed colors */
+int tpss_smoother_clone_shallow(const tpss_smoother *other, const tpss_smoother_desc *desc,
+                                tpss_smoother **out);
+int tpss_smoother_destroy(tpss_smoother *sm);
+/* SchwarzPreconditioner::vmult (schwarz_preconditioner.templates.h:209-239); dst is overwritten */
+int tpss_precond_vmult(tpss_smoother *sm, void *dst, const void *src, void *stream);
+/* SmootherRichardson::step (smoother_richardson.h:118-145) */
+int tpss_smoother_step(tpss_smoother *sm, void *dst, const void *src, void *stream);
+/* SmootherRichardson::vmult (smoother_richardson.h:147-156) */
+int tpss_smoother_vmult(tpss_smoother *sm, void *dst, const void *src, void *stream);
+/* Per-patch damping omega_j replacing the scalar relaxation at
+ * schwarz_preconditioner.templates.h:631-633 (host array of n_patches in the
+ * library's patch order; NULL restores the scalar relaxation). */
+int tpss_smoother_set_patch_damping(tpss_smoother *sm, const double *omega);
+int tpss_smoother_info(const tpss_smoother *sm, int *n_patches, int *n_colors, int *n_dofs_per_patch);
+/* PartitionData::get_patch_range (patch_info.h:763-770): host int[n_colors+1] */
+int tpss_smoother_color_ptr(const tpss_smoother *sm, int *out);
+/* vertex integer coordinates of each patch, host int[n_patches*dim] */
+int tpss_smoother_patch_vertices(const tpss_smoother *sm, int *out);
+/* DoFInfo::dof_indices_patchwise (dof_info.templates.h:156-182): the gather /
+ * scatter index table as the device kernels compute it, host
+ * uint32[n_patches * n_dofs_per_patch] */
+int tpss_smoother_patch_dof_indices(const tpss_smoother *sm, uint32_t *out);
+/* surrogate lengths h[patch][direction][cell_no_1d] (mapping_info.h:126-130), host */
+int tpss_smoother_patch_h(const tpss_smoother *sm, double *out);
+/* SchwarzPreconditioner::get_time_data (schwarz_preconditioner.h:87-199):
+ * sections "Update residuals", "Apply inverses", "Compute inverses",
+ * "Copy Locally Owned", "Step"; *n in: capacity, out: count. */
+int tpss_smoother_get_timings(const tpss_smoother *sm, tpss_time_info *out, int *n);
+int tpss_smoother_set_timing(tpss_smoother *sm, int enabled);
+/* always-on AssertIsFinite (schwarz_preconditioner.templates.h:238 is debug-only
+ * in the reference); costs one device synchronisation per call */
+int tpss_smoother_set_check_finite(tpss_smoother *sm, int enabled);
+
+/* ---- geometric multigrid + CG (caller of the smoother) ------------------- */
+/* ModelProblem::prepare_multigrid + solve (apps/poisson_problem.h:642-755),
+ * MGSmootherSchwarz (apps/multigrid.h:201-399), CoarseGridSolver (:137-197). */
+typedef struct
+{
+  int dim, degree;
+  int n_cells_coarse;      /* cells per direction on the coarsest level */
+  int n_levels;            /* fine level has n_cells_coarse * 2^(n_levels-1) cells */
+  double ip_factor;
+  const double *vertices;  /* fine-level vertices (host) or NULL; coarser
+                              levels take every second vertex */
+  int mg_precision;        /* TPSS_F64 or TPSS_F32: V-cycle vector precision */
+  tpss_smoother_desc smoother; /* pre-smoother; post-smoother = reversed colors */
+  double coarse_tol;       /* relative accuracy of the FP64 coarse CG */
+} tpss_gmg_desc;
+
+typedef struct
+{
+  int iterations;
+  double initial_residual, final_residual;
+  double reduction_rate;   /* rho = (r_n/r_0)^(1/n), apps/postprocess.h:40-60 */
+  double n_frac;           /* log(tol)/log(rho) */
+  double seconds;          /* wall time of the solve (device synchronised) */
+} tpss_solve_info;
+
+int tpss_gmg_create(const tpss_gmg_desc *desc, tpss_gmg **out);
+int tpss_gmg_destroy(tpss_gmg *g);
+int tpss_gmg_level(tpss_gmg *g, int level, tpss_level **out);
+int tpss_gmg_smoother(tpss_gmg *g, int level, int post, tpss_smoother **out);
+/* one V-cycle dst = B src on the fine level (FP64 vectors) */
+int tpss_gmg_vcycle(tpss_gmg *g, void *dst, const void *src, void *stream);
+/* GMG-preconditioned CG in FP64 (x, b: FP64 device vectors of the fine level) */
+int tpss_gmg_solve(tpss_gmg *g, void *x, const void *b, double rel_tol, int max_iter,
+                   void *stream, tpss_solve_info *info, double *residual_history);
+
+/* ---- grid transfer (MGTransferMatrixFree for FE_DGQ) --------------------- */
+int tpss_transfer_prolongate_add(tpss_level *coarse, tpss_level *fine, void *dst_fine,
+                                 const void *src_coarse, void *stream);
+int tpss_transfer_restrict(tpss_level *coarse, tpss_level *fine, void *dst_coarse,
+                           const void *src_fine, void *stream);
+
+/* ---- device memory (plumbing for hosts without their own HIP allocator) --- */
+int tpss_device_count(int *count);
+int tpss_device_malloc(void **ptr, size_t bytes);
+int tpss_device_free(void *ptr);
+int tpss_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int tpss_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int tpss_device_synchronize(void);
+
+/* ---- vectors -------------------------------------------------------------- */
+/* x[i] = U[-1,1] keyed by (seed, i) with SplitMix64 (no reference counterpart:
+ * the reference fills with rand(), generic_functionalities.h:245-278) */
+int tpss_vector_fill_random(void *x, int64_t n, int precision, uint64_t seed, void *stream);
+int tpss_vector_convert(void *dst, int dst_precision, const void *src, int src_precision,
+                        int64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+"""HIP path vs the CPU oracle through the C ABI (needs an MI355X).
+
+Bar: FP64 results within 1e-12 relative (max-norm) of the oracle, index
+tables bit-exact; FP32 patch solves within 1e-5 relative."""
+import numpy as np
+import pytest
+
+import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def _tpss():
+    import tpss_amd
+    try:
+        n = tpss_amd.device_count()
+    except tpss_amd.TPSSError:
+        n = 0
+    if n < 1:
+        pytest.skip("no HIP device")
+    return tpss_amd
+
+
+def dev(a, precision=None):
+    t = _tpss()
+    a = np.ascontiguousarray(a)
+    if precision is None:
+        precision = t.F32 if a.dtype == np.float32 else t.F64
+    return t.DeviceVector.from_numpy(a, precision)
+
+
+def host(v):
+    _tpss().synchronize()
+    return v.numpy().astype(np.float64)
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def make(dim, p, n, variant, coloring=None, relaxation=None, reverse=False, symmetrized=False,
+         damping=1.0, n_steps=1, ip=1.0, precision=0, local_precision=None, vertices=None):
+    import tpss_amd as t
+    if coloring is None:
+        coloring = t.COLOR_NONE if variant == t.ADDITIVE else t.COLOR_REDBLACK
+    if relaxation is None:
+        relaxation = t.DEFAULT_DAMPING[(variant, dim)]
+    lv = t.Level(dim, p, n, ip_factor=ip, precision=precision, vertices=vertices)
+    sm = t.SchwarzSmoother(lv, variant=variant, coloring=coloring, relaxation=relaxation,
+                           reverse=reverse, symmetrized=symmetrized, damping_factor=damping,
+                           n_steps=n_steps, local_precision=local_precision)
+    ref = o.Smoother(dim, p, n, ip=ip, variant=variant, coloring=coloring, relaxation=relaxation,
+                     reverse=reverse, symmetrized=symmetrized, damping=damping, n_steps=n_steps,
+                     vertices=vertices)
+    return lv, sm, ref
+
+
+OPERATOR_CASES = [(2, 1, 5), (2, 3, 8), (2, 4, 6), (3, 1, 4), (3, 2, 5), (3, 3, 6), (3, 4, 3)]
+
+
+@pytest.mark.parametrize("dim,p,n", OPERATOR_CASES)
+def test_operator_parity(dim, p, n):
+    t = _tpss()
+    lv = t.Level(dim, p, n, ip_factor=2.5)
+    ref = o.Smoother(dim, p, n, ip=2.5)
+    x = o.splitmix_uniform(1, ref.N)
+    y = dev(np.zeros_like(x))
+    lv.vmult(y, dev(x))
+    assert rel_err(host(y), ref.operator(x)) < 1e-13
+    b = o.splitmix_uniform(2, ref.N)
+    lv.residual(y, dev(b), dev(x))
+    assert rel_err(host(y), b - ref.operator(x)) < 1e-13
+
+
+@pytest.mark.parametrize("dim,p,n", [(2, 3, 8), (3, 3, 5), (3, 2, 4), (2, 1, 6)])
+def test_patch_index_tables_bit_exact(dim, p, n):
+    import tpss_amd as t
+    for variant in (t.ADDITIVE, t.MULTIPLICATIVE):
+        lv, sm, ref = make(dim, p, n, variant)
+        np.testing.assert_array_equal(sm.color_ptr(), ref.color_ptr())
+        np.testing.assert_array_equal(sm.patch_vertices(), ref.patch_vertices())
+        np.testing.assert_array_equal(sm.patch_dof_indices(), ref.patch_dof_indices())
+
+
+def test_c1_avp_fp64_one_step():
+    """2D SIPG DG Q3 on 128x128, one AVP smoothing step (omega 0.25) on U[-1,1] (seed 1)."""
+    import tpss_amd as t
+    lv, sm, ref = make(2, 3, 128, t.ADDITIVE)
+    src = o.splitmix_uniform(1, ref.N)
+    d = dev(np.zeros(ref.N))
+    sm.precond_vmult(d, dev(src))
+    assert rel_err(host(d), ref.precond_vmult(src)) < 1e-12
+    # smoothing step from zero = smoother vmult
+    sm.vmult(d, dev(src))
+    assert rel_err(host(d), ref.smoother_vmult(src)) < 1e-12
+
+
+@pytest.mark.parametrize("dim,p,n,variant,rev,sym", [
+    (2, 3, 16, 0, False, False), (2, 2, 9, 1, False, False), (2, 3, 10, 1, True, False),
+    (2, 1, 7, 1, False, True), (3, 3, 8, 0, False, False), (3, 3, 8, 1, False, False),
+    (3, 2, 7, 1, True, False), (3, 4, 5, 0, False, False), (3, 1, 6, 1, False, True)])
+def test_precond_parity(dim, p, n, variant, rev, sym):
+    lv, sm, ref = make(dim, p, n, variant, reverse=rev, symmetrized=sym)
+    src = o.splitmix_uniform(3, ref.N)
+    d = dev(np.zeros(ref.N))
+    sm.precond_vmult(d, dev(src))
+    assert rel_err(host(d), ref.precond_vmult(src)) < 1e-12
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_richardson_step_parity(variant):
+    lv, sm, ref = make(3, 2, 6, variant, damping=0.9, n_steps=2)
+    b = o.splitmix_uniform(4, ref.N)
+    x0 = o.splitmix_uniform(5, ref.N)
+    d = dev(x0)
+    sm.step(d, dev(b))
+    assert rel_err(host(d), ref.step(x0.copy(), b)) < 1e-12
+
+
+def test_tiled_coloring_avp_matches_single_color():
+    import tpss_amd as t
+    lv, sm, ref = make(3, 3, 6, t.ADDITIVE, coloring=t.COLOR_TILED)
+    src = o.splitmix_uniform(6, ref.N)
+    d = dev(np.zeros(ref.N))
+    sm.precond_vmult(d, dev(src))
+    ref1 = o.Smoother(3, 3, 6, variant=t.ADDITIVE, relaxation=0.15)
+    assert rel_err(host(d), ref1.precond_vmult(src)) < 1e-12
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_fp32_patch_solves(variant):
+    import tpss_amd as t
+    lv, sm, ref = make(3, 3, 6, variant, precision=t.F32, local_precision=t.F32)
+    src = o.splitmix_uniform(7, ref.N)
+    d = dev(np.zeros(ref.N), t.F32)
+    sm.precond_vmult(d, dev(src, t.F32))
+    assert rel_err(host(d), ref.precond_vmult(src)) < 2e-5
+    # FP32 solves on FP64 vectors
+    lv64, sm64, _ = make(3, 3, 6, variant, local_precision=t.F32)
+    d64 = dev(np.zeros(ref.N))
+    sm64.precond_vmult(d64, dev(src))
+    assert rel_err(host(d64), ref.precond_vmult(src)) < 2e-5
+
+
+def test_per_patch_damping():
+    import tpss_amd as t
+    lv, sm, ref = make(2, 3, 12, t.ADDITIVE)
+    omega = 0.2 + 0.5 * (o.splitmix_uniform(9, sm.n_patches) + 1.0) / 2.0
+    omega = omega.astype(np.float32).astype(np.float64)  # stored as FP32 on the device
+    sm.set_patch_damping(omega)
+    ref.set_patch_damping(omega)
+    src = o.splitmix_uniform(8, ref.N)
+    d = dev(np.zeros(ref.N))
+    sm.precond_vmult(d, dev(src))
+    assert rel_err(host(d), ref.precond_vmult(src)) < 1e-12
+
+
+def test_transfer_parity():
+    import tpss_amd as t
+    for dim, p in ((2, 3), (3, 2), (3, 3)):
+        lc, lf = t.Level(dim, p, 3), t.Level(dim, p, 6)
+        xc = o.splitmix_uniform(1, lc.n_dofs)
+        xf = o.splitmix_uniform(2, lf.n_dofs)
+        d = dev(xf.copy())
+        t.lib.tpss_transfer_prolongate_add(lc.handle, lf.handle, t._ptr(d), t._ptr(dev(xc)), None)
+        ref = xf.copy()
+        o.prolongate_add(dim, p, 3, xc, ref)
+        assert rel_err(host(d), ref) < 1e-14
+        dc = dev(np.zeros(lc.n_dofs))
+        t.lib.tpss_transfer_restrict(lc.handle, lf.handle, t._ptr(dc), t._ptr(dev(xf)), None)
+        assert rel_err(host(dc), o.restrict(dim, p, 3, xf)) < 1e-13
+
+
+def test_random_fill_matches_oracle():
+    t = _tpss()
+    x = t.DeviceVector(10001)
+    t.fill_random(x, seed=1)
+    np.testing.assert_array_equal(host(x), o.splitmix_uniform(1, 10001))
+
+
+@pytest.mark.parametrize("dim,p,levels,variant", [(2, 3, 4, 1), (2, 3, 4, 0), (3, 2, 3, 1),
+                                                   (3, 3, 3, 0)])
+def test_gmg_cg_matches_oracle(dim, p, levels, variant):
+    import tpss_amd as t
+    relax = t.DEFAULT_DAMPING[(variant, dim)]
+    col = t.COLOR_REDBLACK if variant == 1 else t.COLOR_NONE
+    g = t.GMGSolver(dim, p, 2, levels, variant=variant, coloring=col, relaxation=relax)
+    ref = o.GMG(dim, p, 2, levels, variant=variant, coloring=col, relaxation=relax)
+    b = o.splitmix_uniform(2, g.n_dofs)
+    # one V-cycle
+    z = dev(np.zeros(g.n_dofs))
+    g.vcycle(z, dev(b))
+    assert rel_err(host(z), ref.vcycle(b)) < 1e-11
+    x = dev(np.zeros(g.n_dofs))
+    info, hist = g.solve(x, dev(b), rel_tol=1e-8)
+    xr, itr, histr = ref.cg(b, rel_tol=1e-8)
+    assert info.iterations == itr
+    assert rel_err(host(x), xr) < 1e-9
+    np.testing.assert_allclose(hist, histr, rtol=1e-7)
+
+
+def test_gmg_mixed_precision_reaches_tolerance():
+    import tpss_amd as t
+    g = t.GMGSolver(3, 3, 2, 3, mg_precision=t.F32)
+    b = o.splitmix_uniform(2, g.n_dofs)
+    x = dev(np.zeros(g.n_dofs))
+    info, hist = g.solve(x, dev(b), rel_tol=1e-8)
+    assert hist[-1] <= 1e-8 * hist[0]
+    lv = o.Smoother(3, 3, 8)
+    assert np.linalg.norm(b - lv.operator(host(x))) <= 1.01e-8 * np.linalg.norm(b)

@@ -1,0 +1,462 @@
+"""tpss_amd -- MI355X vertex-patch Schwarz smoother (Python host mirror of the C ABI).
+
+The classes mirror the reference interface of the hot path:
+
+* ``Level``                  -- one multigrid level with its SIPG operator
+                               (Laplace::DG::MF::Operator, apps/laplace_integrator.h:578-743)
+* ``SchwarzPreconditioner``  -- ``vmult(dst, src)`` (schwarz_preconditioner.templates.h:209-239)
+* ``SchwarzSmoother``        -- ``vmult`` / ``step`` (smoother_richardson.h:73-156)
+* ``GMGSolver``              -- GMG-preconditioned CG (apps/poisson_problem.h:642-755)
+
+Vectors are device buffers: anything with ``data_ptr()`` (a torch CUDA tensor)
+or a raw integer device address.  All work runs in libtpss_amd.so; there is no
+CPU fallback -- importing raises if the library has not been built.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtpss_amd.so")
+
+OK, E_INVALID_ARG, E_NONFINITE, E_SINGULAR, E_HIP, E_NOT_CONVERGED, E_UNSUPPORTED = range(7)
+ADDITIVE, MULTIPLICATIVE = 0, 1
+COLOR_NONE, COLOR_TILED, COLOR_REDBLACK = 0, 1, 2
+F64, F32, BF16 = 0, 1, 2
+
+#: TPSS::lookup_damping_factor (TPSS.h:120-163) for vertex patches
+DEFAULT_DAMPING = {(ADDITIVE, 2): 0.25, (ADDITIVE, 3): 0.15,
+                   (MULTIPLICATIVE, 2): 1.0, (MULTIPLICATIVE, 3): 1.0}
+
+
+class TPSSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tpss status {code}: {msg}")
+        self.code = code
+
+
+class LevelDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("degree", ctypes.c_int), ("n_cells_1d", ctypes.c_int),
+                ("ip_factor", ctypes.c_double), ("vertices", ctypes.POINTER(ctypes.c_double)),
+                ("vector_precision", ctypes.c_int)]
+
+
+class SmootherDesc(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("coloring", ctypes.c_int),
+                ("relaxation", ctypes.c_double), ("reverse", ctypes.c_int),
+                ("symmetrized", ctypes.c_int), ("damping_factor", ctypes.c_double),
+                ("n_steps", ctypes.c_int), ("local_precision", ctypes.c_int)]
+
+
+class TimeInfo(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("description", ctypes.c_char_p),
+                ("n_calls", ctypes.c_long)]
+
+
+class GMGDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("degree", ctypes.c_int), ("n_cells_coarse", ctypes.c_int),
+                ("n_levels", ctypes.c_int), ("ip_factor", ctypes.c_double),
+                ("vertices", ctypes.POINTER(ctypes.c_double)), ("mg_precision", ctypes.c_int),
+                ("smoother", SmootherDesc), ("coarse_tol", ctypes.c_double)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("initial_residual", ctypes.c_double),
+                ("final_residual", ctypes.c_double), ("reduction_rate", ctypes.c_double),
+                ("n_frac", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+#: every entry point declared in include/tpss.h (checked by the CPU test-suite)
+EXPORTS = [
+    "tpss_last_error", "tpss_version", "tpss_level_create", "tpss_level_destroy",
+    "tpss_level_n_dofs", "tpss_operator_vmult", "tpss_operator_residual",
+    "tpss_smoother_create", "tpss_smoother_clone_shallow", "tpss_smoother_destroy",
+    "tpss_precond_vmult", "tpss_smoother_step", "tpss_smoother_vmult",
+    "tpss_smoother_set_patch_damping", "tpss_smoother_info", "tpss_smoother_color_ptr",
+    "tpss_smoother_patch_vertices", "tpss_smoother_patch_dof_indices", "tpss_smoother_patch_h",
+    "tpss_smoother_get_timings", "tpss_smoother_set_timing", "tpss_smoother_set_check_finite",
+    "tpss_gmg_create", "tpss_gmg_destroy", "tpss_gmg_level", "tpss_gmg_smoother",
+    "tpss_gmg_vcycle", "tpss_gmg_solve", "tpss_transfer_prolongate_add",
+    "tpss_transfer_restrict", "tpss_vector_fill_random", "tpss_vector_convert",
+    "tpss_device_count", "tpss_device_malloc", "tpss_device_free", "tpss_memcpy_h2d",
+    "tpss_memcpy_d2h", "tpss_device_synchronize",
+]
+
+_VP = ctypes.c_void_p
+_PVP = ctypes.POINTER(ctypes.c_void_p)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python tpss_amd/build.py` (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    i64 = ctypes.c_int64
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "tpss_last_error": (ctypes.c_char_p, []),
+        "tpss_version": (ctypes.c_char_p, []),
+        "tpss_level_create": (ctypes.c_int, [ctypes.POINTER(LevelDesc), _PVP]),
+        "tpss_level_destroy": (ctypes.c_int, [_VP]),
+        "tpss_level_n_dofs": (ctypes.c_int, [_VP, ctypes.POINTER(i64)]),
+        "tpss_operator_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_operator_residual": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+        "tpss_smoother_create": (ctypes.c_int, [_VP, ctypes.POINTER(SmootherDesc), _PVP]),
+        "tpss_smoother_clone_shallow": (ctypes.c_int, [_VP, ctypes.POINTER(SmootherDesc), _PVP]),
+        "tpss_smoother_destroy": (ctypes.c_int, [_VP]),
+        "tpss_precond_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_smoother_step": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_smoother_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_smoother_set_patch_damping": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
+        "tpss_smoother_info": (ctypes.c_int, [_VP, ip, ip, ip]),
+        "tpss_smoother_color_ptr": (ctypes.c_int, [_VP, ip]),
+        "tpss_smoother_patch_vertices": (ctypes.c_int, [_VP, ip]),
+        "tpss_smoother_patch_dof_indices": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint32)]),
+        "tpss_smoother_patch_h": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
+        "tpss_smoother_get_timings": (ctypes.c_int, [_VP, ctypes.POINTER(TimeInfo), ip]),
+        "tpss_smoother_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
+        "tpss_smoother_set_check_finite": (ctypes.c_int, [_VP, ctypes.c_int]),
+        "tpss_gmg_create": (ctypes.c_int, [ctypes.POINTER(GMGDesc), _PVP]),
+        "tpss_gmg_destroy": (ctypes.c_int, [_VP]),
+        "tpss_gmg_level": (ctypes.c_int, [_VP, ctypes.c_int, _PVP]),
+        "tpss_gmg_smoother": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _PVP]),
+        "tpss_gmg_vcycle": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_gmg_solve": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_double, ctypes.c_int, _VP,
+                                          ctypes.POINTER(SolveInfo),
+                                          ctypes.POINTER(ctypes.c_double)]),
+        "tpss_transfer_prolongate_add": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+        "tpss_transfer_restrict": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+        "tpss_vector_fill_random": (ctypes.c_int, [_VP, i64, ctypes.c_int, ctypes.c_uint64, _VP]),
+        "tpss_vector_convert": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_int, i64, _VP]),
+        "tpss_device_count": (ctypes.c_int, [ip]),
+        "tpss_device_malloc": (ctypes.c_int, [_PVP, ctypes.c_size_t]),
+        "tpss_device_free": (ctypes.c_int, [_VP]),
+        "tpss_memcpy_h2d": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
+        "tpss_memcpy_d2h": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
+        "tpss_device_synchronize": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(status):
+    if status != OK:
+        raise TPSSError(status, lib.tpss_last_error().decode())
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    return ctypes.c_void_p(x.data_ptr())
+
+
+_DTYPES = {F64: "float64", F32: "float32"}
+
+
+class DeviceVector:
+    """A level vector in device memory allocated by libtpss_amd itself."""
+
+    def __init__(self, n, precision=F64):
+        import numpy as np
+        self.n, self.precision = int(n), precision
+        self.dtype = np.dtype(_DTYPES[precision])
+        p = ctypes.c_void_p()
+        _check(lib.tpss_device_malloc(ctypes.byref(p), self.n * self.dtype.itemsize))
+        self._p = p
+
+    @classmethod
+    def from_numpy(cls, a, precision=None):
+        import numpy as np
+        if precision is None:
+            precision = F32 if np.asarray(a).dtype == np.float32 else F64
+        v = cls(np.asarray(a).size, precision)
+        v.upload(a)
+        return v
+
+    @classmethod
+    def zeros(cls, n, precision=F64):
+        import numpy as np
+        v = cls(n, precision)
+        v.upload(np.zeros(n, dtype=_DTYPES[precision]))
+        return v
+
+    def upload(self, a):
+        import numpy as np
+        h = np.ascontiguousarray(a, dtype=self.dtype).ravel()
+        assert h.size == self.n
+        _check(lib.tpss_memcpy_h2d(self._p, h.ctypes.data_as(ctypes.c_void_p), h.nbytes))
+
+    def numpy(self):
+        import numpy as np
+        h = np.empty(self.n, dtype=self.dtype)
+        _check(lib.tpss_memcpy_d2h(h.ctypes.data_as(ctypes.c_void_p), self._p, h.nbytes))
+        return h
+
+    def data_ptr(self):
+        return self._p.value
+
+    def numel(self):
+        return self.n
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value:
+            lib.tpss_device_free(p)
+            self._p = None
+
+
+def device_count():
+    c = ctypes.c_int()
+    _check(lib.tpss_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def synchronize():
+    _check(lib.tpss_device_synchronize())
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _vertices_ptr(vertices):
+    if vertices is None:
+        return None, None
+    import numpy as np
+    arr = np.ascontiguousarray(vertices, dtype=np.float64)
+    return arr, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _smoother_desc(variant, coloring, relaxation, reverse, symmetrized, damping_factor, n_steps,
+                   local_precision):
+    return SmootherDesc(variant, coloring, float(relaxation), int(reverse), int(symmetrized),
+                        float(damping_factor), int(n_steps), int(local_precision))
+
+
+class Level:
+    """Mesh + DG Q_p space + SIPG operator of one level (device)."""
+
+    def __init__(self, dim, degree, n_cells_1d, ip_factor=1.0, vertices=None, precision=F64):
+        self._varr, vp = _vertices_ptr(vertices)
+        desc = LevelDesc(dim, degree, n_cells_1d, float(ip_factor), vp, precision)
+        h = ctypes.c_void_p()
+        _check(lib.tpss_level_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+        self.dim, self.degree, self.n, self.precision = dim, degree, n_cells_1d, precision
+        n_dofs = ctypes.c_int64()
+        _check(lib.tpss_level_n_dofs(h, ctypes.byref(n_dofs)))
+        self.n_dofs = n_dofs.value
+        self._owned = True
+
+    @classmethod
+    def _borrow(cls, handle, dim, degree, n, precision):
+        self = cls.__new__(cls)
+        self.handle = handle
+        self.dim, self.degree, self.n, self.precision = dim, degree, n, precision
+        n_dofs = ctypes.c_int64()
+        _check(lib.tpss_level_n_dofs(handle, ctypes.byref(n_dofs)))
+        self.n_dofs = n_dofs.value
+        self._owned = False
+        return self
+
+    def vmult(self, dst, src, stream=None):
+        _check(lib.tpss_operator_vmult(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def residual(self, dst, b, x, stream=None):
+        _check(lib.tpss_operator_residual(self.handle, _ptr(dst), _ptr(b), _ptr(x), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self.handle:
+            lib.tpss_level_destroy(self.handle)
+            self.handle = None
+
+
+class SchwarzPreconditioner:
+    """Vertex-patch Schwarz preconditioner of one level (device)."""
+
+    def __init__(self, level, variant=ADDITIVE, coloring=None, relaxation=None, reverse=False,
+                 symmetrized=False, damping_factor=1.0, n_steps=1, local_precision=None,
+                 _handle=None):
+        if coloring is None:
+            coloring = COLOR_NONE if variant == ADDITIVE else COLOR_REDBLACK
+        if relaxation is None:
+            relaxation = DEFAULT_DAMPING[(variant, level.dim)]
+        if local_precision is None:
+            local_precision = level.precision
+        self.level = level
+        self.desc = _smoother_desc(variant, coloring, relaxation, reverse, symmetrized,
+                                   damping_factor, n_steps, local_precision)
+        if _handle is not None:
+            self.handle, self._owned = _handle, False
+        else:
+            h = ctypes.c_void_p()
+            _check(lib.tpss_smoother_create(level.handle, ctypes.byref(self.desc), ctypes.byref(h)))
+            self.handle, self._owned = h, True
+        p, c, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib.tpss_smoother_info(self.handle, ctypes.byref(p), ctypes.byref(c), ctypes.byref(k)))
+        self.n_patches, self.n_colors, self.n_dofs_per_patch = p.value, c.value, k.value
+
+    def shallow_copy(self, reverse=None, symmetrized=None, relaxation=None):
+        d = SmootherDesc.from_buffer_copy(self.desc)
+        if reverse is not None:
+            d.reverse = int(reverse)
+        if symmetrized is not None:
+            d.symmetrized = int(symmetrized)
+        if relaxation is not None:
+            d.relaxation = float(relaxation)
+        h = ctypes.c_void_p()
+        _check(lib.tpss_smoother_clone_shallow(self.handle, ctypes.byref(d), ctypes.byref(h)))
+        other = type(self).__new__(type(self))
+        other.level, other.desc, other.handle, other._owned = self.level, d, h, True
+        other.n_patches, other.n_colors, other.n_dofs_per_patch = (
+            self.n_patches, self.n_colors, self.n_dofs_per_patch)
+        other._parent = self
+        return other
+
+    def vmult(self, dst, src, stream=None):
+        _check(lib.tpss_precond_vmult(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def set_patch_damping(self, omega):
+        if omega is None:
+            _check(lib.tpss_smoother_set_patch_damping(self.handle, None))
+            return
+        import numpy as np
+        w = np.ascontiguousarray(omega, dtype=np.float64)
+        assert w.shape == (self.n_patches,)
+        _check(lib.tpss_smoother_set_patch_damping(self.handle,
+                                                   w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def set_timing(self, enabled=True):
+        _check(lib.tpss_smoother_set_timing(self.handle, int(enabled)))
+
+    def set_check_finite(self, enabled=True):
+        _check(lib.tpss_smoother_set_check_finite(self.handle, int(enabled)))
+
+    def get_time_data(self):
+        n = ctypes.c_int(8)
+        arr = (TimeInfo * 8)()
+        _check(lib.tpss_smoother_get_timings(self.handle, arr, ctypes.byref(n)))
+        return [(arr[i].description.decode(), arr[i].time, arr[i].n_calls) for i in range(n.value)]
+
+    def color_ptr(self):
+        import numpy as np
+        out = np.zeros(self.n_colors + 1, dtype=np.int32)
+        _check(lib.tpss_smoother_color_ptr(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
+
+    def patch_vertices(self):
+        import numpy as np
+        out = np.zeros((self.n_patches, self.level.dim), dtype=np.int32)
+        _check(lib.tpss_smoother_patch_vertices(self.handle,
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
+
+    def patch_dof_indices(self):
+        import numpy as np
+        out = np.zeros((self.n_patches, self.n_dofs_per_patch), dtype=np.uint32)
+        _check(lib.tpss_smoother_patch_dof_indices(
+            self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    def patch_h(self):
+        import numpy as np
+        out = np.zeros((self.n_patches, self.level.dim, 2))
+        _check(lib.tpss_smoother_patch_h(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self.handle:
+            lib.tpss_smoother_destroy(self.handle)
+            self.handle = None
+
+
+class SchwarzSmoother(SchwarzPreconditioner):
+    """SchwarzSmoother = SmootherRichardson around the preconditioner:
+    ``step(dst, src)``: dst += omega_R * P (src - A dst), ``vmult``: dst = 0; step."""
+
+    def step(self, dst, src, stream=None):
+        _check(lib.tpss_smoother_step(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def vmult(self, dst, src, stream=None):  # noqa: D401 -- SmootherRichardson::vmult
+        _check(lib.tpss_smoother_vmult(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def precond_vmult(self, dst, src, stream=None):
+        _check(lib.tpss_precond_vmult(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+
+class GMGSolver:
+    """GMG-preconditioned CG in FP64 with Schwarz smoothers on every level."""
+
+    def __init__(self, dim, degree, n_cells_coarse, n_levels, ip_factor=1.0, vertices=None,
+                 mg_precision=F64, variant=MULTIPLICATIVE, coloring=None, relaxation=None,
+                 damping_factor=1.0, n_steps=1, local_precision=None, coarse_tol=1e-12):
+        if coloring is None:
+            coloring = COLOR_NONE if variant == ADDITIVE else COLOR_REDBLACK
+        if relaxation is None:
+            relaxation = DEFAULT_DAMPING[(variant, dim)]
+        if local_precision is None:
+            local_precision = mg_precision
+        self._varr, vp = _vertices_ptr(vertices)
+        sd = _smoother_desc(variant, coloring, relaxation, False, False, damping_factor, n_steps,
+                            local_precision)
+        self.desc = GMGDesc(dim, degree, n_cells_coarse, n_levels, float(ip_factor), vp,
+                            mg_precision, sd, float(coarse_tol))
+        h = ctypes.c_void_p()
+        _check(lib.tpss_gmg_create(ctypes.byref(self.desc), ctypes.byref(h)))
+        self.handle = h
+        self.dim, self.degree, self.n_levels = dim, degree, n_levels
+        self.n_fine = n_cells_coarse << (n_levels - 1)
+        self.n_dofs = self.n_fine ** dim * (degree + 1) ** dim
+        self.mg_precision = mg_precision
+
+    def level(self, l):
+        h = ctypes.c_void_p()
+        _check(lib.tpss_gmg_level(self.handle, l, ctypes.byref(h)))
+        return Level._borrow(h, self.dim, self.degree, self.desc.n_cells_coarse << l,
+                             self.mg_precision)
+
+    def smoother(self, l, post=False):
+        h = ctypes.c_void_p()
+        _check(lib.tpss_gmg_smoother(self.handle, l, int(post), ctypes.byref(h)))
+        return SchwarzSmoother(self.level(l), _handle=h, variant=self.desc.smoother.variant,
+                               coloring=self.desc.smoother.coloring,
+                               relaxation=self.desc.smoother.relaxation,
+                               local_precision=self.desc.smoother.local_precision)
+
+    def vcycle(self, dst, src, stream=None):
+        _check(lib.tpss_gmg_vcycle(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def solve(self, x, b, rel_tol=1e-8, max_iter=200, stream=None, raise_on_fail=True):
+        info = SolveInfo()
+        hist = (ctypes.c_double * (max_iter + 2))()
+        st = lib.tpss_gmg_solve(self.handle, _ptr(x), _ptr(b), float(rel_tol), int(max_iter),
+                                _stream(stream), ctypes.byref(info), hist)
+        if st != OK and (raise_on_fail or st != E_NOT_CONVERGED):
+            _check(st)
+        n = info.iterations if info.iterations >= 0 else max_iter
+        return info, [hist[i] for i in range(n + 1)]
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.tpss_gmg_destroy(self.handle)
+            self.handle = None
+
+
+def fill_random(x, seed, precision=None, stream=None):
+    if precision is None:
+        precision = getattr(x, "precision", F64)
+    _check(lib.tpss_vector_fill_random(_ptr(x), x.numel(), precision, seed, _stream(stream)))
+
+
+def version():
+    return lib.tpss_version().decode()

@@ -1,0 +1,333 @@
+// capi.hip -- the C ABI of libtpss_amd: levels (SIPG operator), Schwarz
+// smoothers, grid transfer, vectors, geometric multigrid + CG.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <type_traits>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "setup_kernels.hpp"
+#include "ops.hpp"
+
+namespace tpss
+{
+const char *last_error_cstr();
+
+
+// ---------------------------------------------------------------------------- levels
+
+template<typename T, int NP>
+OpTab<T, NP> make_optab(const Operator1D &op)
+{
+  OpTab<T, NP> t;
+  for(int e = 0; e < NP * NP; ++e)
+  {
+    t.M[e] = T(op.M[e]);
+    t.Fpm[e] = T(op.Fpm[e]);
+    t.Fmp[e] = T(op.Fmp[e]);
+    for(int c = 0; c < 4; ++c) t.D[c][e] = T(op.D[c][e]);
+  }
+  return t;
+}
+
+template<int DIM, int NP>
+constexpr int cells_per_block()
+{
+  constexpr int ndc = cpow(NP, DIM);
+  return ndc >= 256 ? 1 : 256 / ndc;
+}
+
+// y = A x (or y = b - A x), optionally only on a list of cells; vectors of type VT
+template<typename VT>
+void operator_apply(const tpss_level *lv, VT *y, const VT *x, const VT *b, const uint32_t *cells,
+                    int64_t n_list, hipStream_t s)
+{
+  if(lv->distorted)
+    throw Error(TPSS_E_UNSUPPORTED, "SIPG operator on distorted meshes is not available on the device");
+  dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
+    constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
+    constexpr int CPB = cells_per_block<DIM, NP>();
+    constexpr int NDC = cpow(NP, DIM);
+    using CT = VT;
+    const auto tab = make_optab<CT, NP>(lv->op1d);
+    const int64_t count = cells ? n_list : lv->ncell;
+    if(count == 0) return;
+    const unsigned grid = blocks_for(count, CPB);
+    const unsigned threads = unsigned(((CPB * NDC + 63) / 64) * 64);
+    if(b)
+      hipLaunchKernelGGL((k_sipg_cartesian<DIM, NP, VT, CT, CPB, true>), dim3(grid), dim3(threads),
+                         0, s, x, y, b, tab, lv->n, lv->ncell, cells, n_list);
+    else
+      hipLaunchKernelGGL((k_sipg_cartesian<DIM, NP, VT, CT, CPB, false>), dim3(grid),
+                         dim3(threads), 0, s, x, y, b, tab, lv->n, lv->ncell, cells, n_list);
+    TPSS_HIP(hipGetLastError());
+  });
+}
+
+void operator_apply_any(const tpss_level *lv, void *y, const void *x, const void *b,
+                        const uint32_t *cells, int64_t n_list, hipStream_t s)
+{
+  if(lv->vec_prec == TPSS_F64)
+    operator_apply<double>(lv, (double *)y, (const double *)x, (const double *)b, cells, n_list, s);
+  else
+    operator_apply<float>(lv, (float *)y, (const float *)x, (const float *)b, cells, n_list, s);
+}
+
+size_t elem_size(int prec) { return prec == TPSS_F64 ? 8 : 4; }
+
+template<typename T>
+void fill(T *x, int64_t n, T v, hipStream_t s)
+{
+  if(n == 0) return;
+  hipLaunchKernelGGL((k_fill<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, x, n, v);
+  TPSS_HIP(hipGetLastError());
+}
+
+void fill_zero(void *x, int prec, int64_t n, hipStream_t s)
+{
+  if(prec == TPSS_F64)
+    fill<double>((double *)x, n, 0.0, s);
+  else
+    fill<float>((float *)x, n, 0.0f, s);
+}
+
+// y = a x + b y
+void axpby(void *y, const void *x, int prec, int64_t n, double a, double b, hipStream_t s)
+{
+  if(n == 0) return;
+  if(prec == TPSS_F64)
+    hipLaunchKernelGGL((k_axpby<double>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (double *)y,
+                       (const double *)x, n, a, b);
+  else
+    hipLaunchKernelGGL((k_axpby<float>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (float *)y,
+                       (const float *)x, n, a, b);
+  TPSS_HIP(hipGetLastError());
+}
+
+void convert(void *dst, int dp, const void *src, int sp, int64_t n, hipStream_t s)
+{
+  if(n == 0) return;
+  const unsigned g = blocks_for(n, 256);
+  if(dp == TPSS_F64 && sp == TPSS_F64)
+    TPSS_HIP(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, s));
+  else if(dp == TPSS_F32 && sp == TPSS_F32)
+    TPSS_HIP(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, s));
+  else if(dp == TPSS_F64 && sp == TPSS_F32)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(256), 0, s, (double *)dst,
+                       (const float *)src, n);
+  else if(dp == TPSS_F32 && sp == TPSS_F64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(256), 0, s, (float *)dst,
+                       (const double *)src, n);
+  else
+    throw Error(TPSS_E_INVALID_ARG, "unsupported vector precision");
+  TPSS_HIP(hipGetLastError());
+}
+
+}  // namespace tpss
+
+using namespace tpss;
+
+extern "C" {
+
+const char *tpss_last_error(void) { return last_error_cstr(); }
+const char *tpss_version(void) { return "tpss_amd 0.1 (gfx950)"; }
+
+int tpss_level_create(const tpss_level_desc *desc, tpss_level **out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(desc && out, "null argument");
+    TPSS_REQUIRE(desc->dim == 2 || desc->dim == 3, "dim must be 2 or 3");
+    TPSS_REQUIRE(desc->degree >= 1 && desc->degree <= 4, "degree must be in 1..4");
+    TPSS_REQUIRE(desc->n_cells_1d >= 2 && desc->n_cells_1d <= 1000, "n_cells_1d must be in 2..1000");
+    TPSS_REQUIRE(desc->ip_factor > 0, "ip_factor must be positive");
+    TPSS_REQUIRE(desc->vector_precision == TPSS_F64 || desc->vector_precision == TPSS_F32,
+                 "vector precision must be F64 or F32");
+    auto lv = std::make_unique<tpss_level>();
+    lv->dim = desc->dim;
+    lv->p = desc->degree;
+    lv->np = desc->degree + 1;
+    lv->n = desc->n_cells_1d;
+    lv->ndc = ipow(lv->np, lv->dim);
+    lv->ncell = ipow(lv->n, lv->dim);
+    lv->N = lv->ncell * lv->ndc;
+    lv->ip = desc->ip_factor;
+    lv->h = 1.0 / lv->n;
+    lv->vec_prec = desc->vector_precision;
+    if(desc->vertices)
+    {
+      const size_t nv = size_t(ipow(lv->n + 1, lv->dim)) * lv->dim;
+      lv->vertices.assign(desc->vertices, desc->vertices + nv);
+      lv->d_vertices.upload(lv->vertices);
+      lv->distorted = true;
+    }
+    lv->op1d = make_operator_1d(lv->p, lv->h, lv->ip);
+    *out = lv.release();
+  });
+}
+
+int tpss_level_destroy(tpss_level *level)
+{
+  delete level;
+  return TPSS_OK;
+}
+
+int tpss_level_n_dofs(const tpss_level *level, int64_t *n_dofs)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(level && n_dofs, "null argument");
+    *n_dofs = level->N;
+  });
+}
+
+int tpss_operator_vmult(tpss_level *level, void *dst, const void *src, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(level && dst && src, "null argument");
+    operator_apply_any(level, dst, src, nullptr, nullptr, 0, as_stream(stream));
+  });
+}
+
+int tpss_operator_residual(tpss_level *level, void *dst, const void *b, const void *x, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(level && dst && b && x, "null argument");
+    operator_apply_any(level, dst, x, b, nullptr, 0, as_stream(stream));
+  });
+}
+
+int tpss_device_count(int *count)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(count, "null argument");
+    TPSS_HIP(hipGetDeviceCount(count));
+  });
+}
+
+int tpss_device_malloc(void **ptr, size_t bytes)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(ptr, "null argument");
+    *ptr = nullptr;
+    if(bytes) TPSS_HIP(hipMalloc(ptr, bytes));
+  });
+}
+
+int tpss_device_free(void *ptr)
+{
+  return guarded([&] {
+    if(ptr) TPSS_HIP(hipFree(ptr));
+  });
+}
+
+int tpss_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+  return guarded([&] {
+    TPSS_REQUIRE((dst && src) || bytes == 0, "null argument");
+    if(bytes) TPSS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int tpss_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+  return guarded([&] {
+    TPSS_REQUIRE((dst && src) || bytes == 0, "null argument");
+    if(bytes) TPSS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int tpss_device_synchronize(void)
+{
+  return guarded([&] { TPSS_HIP(hipDeviceSynchronize()); });
+}
+
+int tpss_vector_fill_random(void *x, int64_t n, int precision, uint64_t seed, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(x || n == 0, "null argument");
+    if(n == 0) return;
+    hipStream_t s = as_stream(stream);
+    if(precision == TPSS_F64)
+      hipLaunchKernelGGL((k_random<double>), dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                         (double *)x, n, seed);
+    else if(precision == TPSS_F32)
+      hipLaunchKernelGGL((k_random<float>), dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                         (float *)x, n, seed);
+    else
+      throw Error(TPSS_E_INVALID_ARG, "precision must be F64 or F32");
+    TPSS_HIP(hipGetLastError());
+  });
+}
+
+int tpss_vector_convert(void *dst, int dst_precision, const void *src, int src_precision,
+                        int64_t n, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE((dst && src) || n == 0, "null argument");
+    convert(dst, dst_precision, src, src_precision, n, as_stream(stream));
+  });
+}
+
+int tpss_transfer_prolongate_add(tpss_level *coarse, tpss_level *fine, void *dst_fine,
+                                 const void *src_coarse, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(coarse && fine && dst_fine && src_coarse, "null argument");
+    TPSS_REQUIRE(fine->n == 2 * coarse->n && fine->dim == coarse->dim && fine->p == coarse->p &&
+                   fine->vec_prec == coarse->vec_prec,
+                 "levels are not a refinement pair");
+    hipStream_t s = as_stream(stream);
+    dispatch_dim_np(fine->dim, fine->np, [&](auto D, auto NPc) {
+      constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
+      TransferTab<NP> tab;
+      double P[2 * MAXNP * MAXNP];
+      prolongation_1d(NP - 1, P);
+      for(int c = 0; c < 2; ++c)
+        for(int e = 0; e < NP * NP; ++e) tab.P[c][e] = P[c * NP * NP + e];
+      const unsigned g = blocks_for(fine->N, 256);
+      if(fine->vec_prec == TPSS_F64)
+        hipLaunchKernelGGL((k_prolongate_add<DIM, NP, double>), dim3(g), dim3(256), 0, s,
+                           (double *)dst_fine, (const double *)src_coarse, tab, coarse->n, fine->N);
+      else
+        hipLaunchKernelGGL((k_prolongate_add<DIM, NP, float>), dim3(g), dim3(256), 0, s,
+                           (float *)dst_fine, (const float *)src_coarse, tab, coarse->n, fine->N);
+      TPSS_HIP(hipGetLastError());
+    });
+  });
+}
+
+int tpss_transfer_restrict(tpss_level *coarse, tpss_level *fine, void *dst_coarse,
+                           const void *src_fine, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(coarse && fine && dst_coarse && src_fine, "null argument");
+    TPSS_REQUIRE(fine->n == 2 * coarse->n && fine->dim == coarse->dim && fine->p == coarse->p &&
+                   fine->vec_prec == coarse->vec_prec,
+                 "levels are not a refinement pair");
+    hipStream_t s = as_stream(stream);
+    dispatch_dim_np(fine->dim, fine->np, [&](auto D, auto NPc) {
+      constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
+      TransferTab<NP> tab;
+      double P[2 * MAXNP * MAXNP];
+      prolongation_1d(NP - 1, P);
+      for(int c = 0; c < 2; ++c)
+        for(int e = 0; e < NP * NP; ++e) tab.P[c][e] = P[c * NP * NP + e];
+      const unsigned g = blocks_for(coarse->N, 256);
+      if(fine->vec_prec == TPSS_F64)
+        hipLaunchKernelGGL((k_restrict<DIM, NP, double>), dim3(g), dim3(256), 0, s,
+                           (double *)dst_coarse, (const double *)src_fine, tab, coarse->n, coarse->N);
+      else
+        hipLaunchKernelGGL((k_restrict<DIM, NP, float>), dim3(g), dim3(256), 0, s,
+                           (float *)dst_coarse, (const float *)src_fine, tab, coarse->n, coarse->N);
+      TPSS_HIP(hipGetLastError());
+    });
+  });
+}
+
+}  // extern "C"

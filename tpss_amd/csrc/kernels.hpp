@@ -1,0 +1,656 @@
+// kernels.hpp -- device kernels of libtpss_amd (gfx950).
+//
+// Data layout in HBM (shared by every kernel):
+//   level vector   x[cell * (p+1)^dim + local], cell = sum_d c_d n^d (x fastest),
+//                  local lexicographic (x fastest) on Gauss-Lobatto nodes;
+//   patch tables   vtx[j] = v_0 | v_1 << 10 | v_2 << 20 (interior vertex),
+//                  fac[j*dim + d] = row of the 1D factor table for direction d;
+//   factor table   S[f][a][i] (eigenvector i in column i), lam[f][i];
+//   AVP buffer     U[(cell * 2^dim + slot) * (p+1)^dim + local], slot = position
+//                  of the cell inside the patch that produced the value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tpss
+{
+constexpr __host__ __device__ int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+
+// ----------------------------------------------------------------------------
+// SIPG operator on the Cartesian mesh (apps/laplace_integrator.h:578-743 for
+// tensor-product cells): per cell and direction d the 1D line operator
+//   z = D_d x_c + F_pm x_{c-e_d} + F_mp x_{c+e_d}   (along d)
+// followed by the cell mass along every other direction, summed over d.
+// One thread per cell DoF, CPB cells per block, lines staged in LDS.
+// ----------------------------------------------------------------------------
+template<typename T, int NP>
+struct OpTab
+{
+  T M[NP * NP];
+  T D[4][NP * NP];
+  T Fpm[NP * NP];
+  T Fmp[NP * NP];
+};
+
+template<int DIM, int NP, typename VT, typename CT, int CPB, bool RESIDUAL>
+__global__ void __launch_bounds__(256)
+k_sipg_cartesian(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
+                 const OpTab<CT, NP> tab, const int n, const int64_t ncell,
+                 const uint32_t *__restrict__ cells, const int64_t n_list)
+{
+  constexpr int NDC = cpow(NP, DIM);
+  __shared__ CT xs[CPB][NDC];
+  __shared__ CT zs[CPB][NDC];
+  const int cs = threadIdx.x / NDC, li = threadIdx.x % NDC;
+  const int64_t slot = int64_t(blockIdx.x) * CPB + cs;
+  const int64_t count = cells ? n_list : ncell;
+  const bool active = cs < CPB && slot < count;
+  const int64_t cell = active ? (cells ? int64_t(cells[slot]) : slot) : 0;
+  const int csafe = cs < CPB ? cs : 0;
+
+  int lid[DIM], cid[DIM];
+  {
+    int t = li;
+    int64_t c = cell;
+    for(int d = 0; d < DIM; ++d)
+    {
+      lid[d] = t % NP;
+      t /= NP;
+      cid[d] = int(c % n);
+      c /= n;
+    }
+  }
+  if(cs < CPB) xs[cs][li] = active ? CT(x[cell * NDC + li]) : CT(0);
+  __syncthreads();
+
+  CT acc = 0;
+  int64_t cstride = 1;
+  int lstride = 1;
+#pragma unroll
+  for(int d = 0; d < DIM; ++d)
+  {
+    const bool has_l = cid[d] > 0, has_r = cid[d] < n - 1;
+    const CT *D = tab.D[(has_l ? 1 : 0) | (has_r ? 2 : 0)];
+    const int i = lid[d];
+    const int base = li - i * lstride;
+    CT z = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j) z += D[i * NP + j] * xs[csafe][base + j * lstride];
+    if(active && has_l)
+    {
+      const VT *xl = x + (cell - cstride) * NDC + base;
+#pragma unroll
+      for(int j = 0; j < NP; ++j) z += tab.Fpm[i * NP + j] * CT(xl[j * lstride]);
+    }
+    if(active && has_r)
+    {
+      const VT *xr = x + (cell + cstride) * NDC + base;
+#pragma unroll
+      for(int j = 0; j < NP; ++j) z += tab.Fmp[i * NP + j] * CT(xr[j * lstride]);
+    }
+    int estride = 1;
+#pragma unroll
+    for(int e = 0; e < DIM; ++e)
+    {
+      if(e != d)
+      {
+        __syncthreads();
+        if(cs < CPB) zs[cs][li] = z;
+        __syncthreads();
+        const int ie = lid[e];
+        const int be = li - ie * estride;
+        CT w = 0;
+#pragma unroll
+        for(int j = 0; j < NP; ++j) w += tab.M[ie * NP + j] * zs[csafe][be + j * estride];
+        z = w;
+      }
+      estride *= NP;
+    }
+    acc += z;
+    cstride *= n;
+    lstride *= NP;
+  }
+  if(active)
+  {
+    if(RESIDUAL)
+      y[cell * NDC + li] = VT(CT(b[cell * NDC + li]) - acc);
+    else
+      y[cell * NDC + li] = VT(acc);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Fused vertex-patch local solve (A2/A3/A4/A5/A12):
+//   gather r_j = R_j src  ->  u_j = S diag(omega_j / sum lambda) S^T r_j  ->
+//   MODE 0: dst += R_j^T u_j   (multiplicative color pass; same-color patches
+//                               are DoF-disjoint so the update is race free)
+//   MODE 1: U[cell, slot] = u_j (additive; summed per DoF afterwards, no atomics)
+// Thread = one line of the patch tensor; the line direction rotates through
+// LDS between the per-direction contractions.
+// ----------------------------------------------------------------------------
+template<int DIM, int K>
+struct PatchShape
+{
+  static constexpr int NP = K / 2;
+  static constexpr int L = cpow(K, DIM - 1);   // lines per patch
+  static constexpr int NPD = cpow(K, DIM);     // DoFs per patch
+  static constexpr int NDC = cpow(NP, DIM);    // DoFs per cell
+  static constexpr int PPB = (128 / L) > 0 ? (128 / L) : 1;
+  static constexpr int THREADS = ((PPB * L + 63) / 64) * 64;  // launched block size
+};
+
+inline __device__ __forceinline__ void unpack_vertex(uint32_t w, int *v)
+{
+  v[0] = int(w & 1023u);
+  v[1] = int((w >> 10) & 1023u);
+  v[2] = int((w >> 20) & 1023u);
+}
+
+// Patch-local multi-index (i0, i1, i2), i_d in [0, 2(p+1)), of the patch at
+// interior vertex v -> global cell, cell-local DoF and the cell's position in
+// the patch.  The DGQ 1D map i_d = cell_no*(p+1) + local (dof_info.h:358-369)
+// with cells of the patch in lexicographic order (patch_info.templates.h:401-402).
+template<int DIM, int NP>
+__device__ __forceinline__ int64_t patch_cell(const int *v, int n, int i0, int i1, int i2)
+{
+  int64_t c = v[0] - 1 + (i0 >= NP ? 1 : 0);
+  if(DIM > 1) c += int64_t(n) * (v[1] - 1 + (i1 >= NP ? 1 : 0));
+  if(DIM > 2) c += int64_t(n) * n * (v[2] - 1 + (i2 >= NP ? 1 : 0));
+  return c;
+}
+
+template<int DIM, int NP>
+__device__ __forceinline__ int patch_local(int i0, int i1, int i2)
+{
+  int l = i0 >= NP ? i0 - NP : i0;
+  if(DIM > 1) l += NP * (i1 >= NP ? i1 - NP : i1);
+  if(DIM > 2) l += NP * NP * (i2 >= NP ? i2 - NP : i2);
+  return l;
+}
+
+template<int DIM, int NP>
+__device__ __forceinline__ int patch_slot(int i0, int i1, int i2)
+{
+  int s = (i0 >= NP ? 1 : 0);
+  if(DIM > 1) s |= (i1 >= NP ? 2 : 0);
+  if(DIM > 2) s |= (i2 >= NP ? 4 : 0);
+  return s;
+}
+
+// the gather/scatter index table: out[j * K^dim + i] = global DoF of patch DoF i
+template<int DIM, int K>
+__global__ void k_patch_dof_indices(const uint32_t *__restrict__ vtx, const int P, const int n,
+                                    uint32_t *__restrict__ out)
+{
+  constexpr int NP = K / 2, NPD = cpow(K, DIM), NDC = cpow(NP, DIM);
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(g >= int64_t(P) * NPD) return;
+  const int j = int(g / NPD), i = int(g % NPD);
+  int v[3];
+  unpack_vertex(vtx[j], v);
+  const int i0 = i % K, i1 = DIM > 1 ? (i / K) % K : 0, i2 = DIM > 2 ? i / (K * K) : 0;
+  out[g] = uint32_t(patch_cell<DIM, NP>(v, n, i0, i1, i2) * NDC + patch_local<DIM, NP>(i0, i1, i2));
+}
+
+template<int DIM, int K, typename VT, typename CT, int MODE>
+__global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS))
+k_patch_fd(const VT *__restrict__ src, VT *__restrict__ dst, CT *__restrict__ ubuf,
+           const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
+           const CT *__restrict__ S_tab, const CT *__restrict__ lam_tab,
+           const float *__restrict__ omega, const CT relaxation, const int patch_begin,
+           const int patch_end, const int n)
+{
+  using PS = PatchShape<DIM, K>;
+  constexpr int NP = PS::NP, L = PS::L, NPD = PS::NPD, NDC = PS::NDC, PPB = PS::PPB;
+  __shared__ CT buf[PPB][NPD];
+  __shared__ CT Ss[PPB][DIM][K * K];
+  __shared__ CT ls[PPB][DIM][K];
+
+  const int pl = threadIdx.x / L, t = threadIdx.x % L;
+  const int j0 = patch_begin + int(blockIdx.x) * PPB;
+  const int j = j0 + pl;
+  const bool valid = pl < PPB && j < patch_end;
+  const int plc = pl < PPB ? pl : 0;
+
+  for(int idx = threadIdx.x; idx < PPB * DIM * K * K; idx += blockDim.x)
+  {
+    const int q = idx / (DIM * K * K), r = idx % (DIM * K * K);
+    const int d = r / (K * K), e = r % (K * K);
+    const int jj = j0 + q;
+    Ss[q][d][e] = jj < patch_end ? S_tab[size_t(fac[size_t(jj) * DIM + d]) * K * K + e] : CT(0);
+  }
+  for(int idx = threadIdx.x; idx < PPB * DIM * K; idx += blockDim.x)
+  {
+    const int q = idx / (DIM * K), r = idx % (DIM * K);
+    const int d = r / K, e = r % K;
+    const int jj = j0 + q;
+    ls[q][d][e] = jj < patch_end ? lam_tab[size_t(fac[size_t(jj) * DIM + d]) * K + e] : CT(1);
+  }
+  int v[3] = {1, 1, 1};
+  if(valid) unpack_vertex(vtx[j], v);
+  const CT w = valid ? (omega ? CT(omega[j]) : relaxation) : CT(0);
+  __syncthreads();
+
+  auto cell_of = [&](int i0, int i1, int i2) { return patch_cell<DIM, NP>(v, n, i0, i1, i2); };
+  auto loc_of = [&](int i0, int i1, int i2) { return patch_local<DIM, NP>(i0, i1, i2); };
+  auto slot_of = [&](int i0, int i1, int i2) { return patch_slot<DIM, NP>(i0, i1, i2); };
+
+  CT r[K], o[K];
+  const CT *S0 = Ss[plc][0];
+  const CT *S1 = Ss[plc][1];
+  const CT *S2 = Ss[plc][DIM > 2 ? 2 : 1];
+  CT *B = buf[plc];
+
+  // ---- gather x-lines: thread (y, z) = (t % K, t / K)
+  const int ty = DIM > 1 ? t % K : 0;
+  const int tz = DIM > 2 ? t / K : 0;
+#pragma unroll
+  for(int i = 0; i < K; ++i)
+    r[i] = valid ? CT(src[cell_of(i, ty, tz) * NDC + loc_of(i, ty, tz)]) : CT(0);
+
+  // ---- forward x
+#pragma unroll
+  for(int i = 0; i < K; ++i)
+  {
+    CT s = 0;
+#pragma unroll
+    for(int a = 0; a < K; ++a) s += S0[a * K + i] * r[a];
+    o[i] = s;
+  }
+  if(DIM == 2)
+  {
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+      if(pl < PPB) B[i + K * ty] = o[i];
+    __syncthreads();
+    const int x0 = t;  // y-line at x = t
+#pragma unroll
+    for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj];
+#pragma unroll
+    for(int jj = 0; jj < K; ++jj)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
+      o[jj] = s * w / (ls[plc][0][x0] + ls[plc][1][jj]);
+    }
+#pragma unroll
+    for(int b = 0; b < K; ++b)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj) s += S1[b * K + jj] * o[jj];
+      r[b] = s;
+    }
+#pragma unroll
+    for(int b = 0; b < K; ++b)
+      if(pl < PPB) B[x0 + K * b] = r[b];
+    __syncthreads();
+#pragma unroll
+    for(int i = 0; i < K; ++i) r[i] = B[i + K * ty];
+  }
+  else
+  {
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+      if(pl < PPB) B[i + K * ty + K * K * tz] = o[i];
+    __syncthreads();
+    // y-lines: thread (x, z) = (t % K, t / K)
+    {
+      const int x0 = t % K, z0 = t / K;
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj + K * K * z0];
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
+        o[jj] = s;
+      }
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj)
+        if(pl < PPB) B[x0 + K * jj + K * K * z0] = o[jj];
+    }
+    __syncthreads();
+    // z-lines: thread (x, y) = (t % K, t / K)
+    {
+      const int x0 = t % K, y0 = t / K;
+#pragma unroll
+      for(int l = 0; l < K; ++l) r[l] = B[x0 + K * y0 + K * K * l];
+      const CT lxy = ls[plc][0][x0] + ls[plc][1][y0];
+#pragma unroll
+      for(int l = 0; l < K; ++l)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int c = 0; c < K; ++c) s += S2[c * K + l] * r[c];
+        o[l] = s * w / (lxy + ls[plc][2][l]);
+      }
+#pragma unroll
+      for(int c = 0; c < K; ++c)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int l = 0; l < K; ++l) s += S2[c * K + l] * o[l];
+        r[c] = s;
+      }
+#pragma unroll
+      for(int c = 0; c < K; ++c)
+        if(pl < PPB) B[x0 + K * y0 + K * K * c] = r[c];
+    }
+    __syncthreads();
+    // backward y
+    {
+      const int x0 = t % K, z0 = t / K;
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj + K * K * z0];
+#pragma unroll
+      for(int b = 0; b < K; ++b)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int jj = 0; jj < K; ++jj) s += S1[b * K + jj] * r[jj];
+        o[b] = s;
+      }
+#pragma unroll
+      for(int b = 0; b < K; ++b)
+        if(pl < PPB) B[x0 + K * b + K * K * z0] = o[b];
+    }
+    __syncthreads();
+#pragma unroll
+    for(int i = 0; i < K; ++i) r[i] = B[i + K * ty + K * K * tz];
+  }
+
+  // ---- backward x and scatter
+#pragma unroll
+  for(int a = 0; a < K; ++a)
+  {
+    CT s = 0;
+#pragma unroll
+    for(int i = 0; i < K; ++i) s += S0[a * K + i] * r[i];
+    o[a] = s;
+  }
+  if(!valid) return;
+#pragma unroll
+  for(int a = 0; a < K; ++a)
+  {
+    const int64_t c = cell_of(a, ty, tz);
+    const int l = loc_of(a, ty, tz);
+    if(MODE == 0)
+    {
+      VT *p = dst + c * NDC + l;
+      *p = VT(CT(*p) + o[a]);
+    }
+    else
+    {
+      ubuf[(c * (1 << DIM) + slot_of(a, ty, tz)) * NDC + l] = o[a];
+    }
+  }
+}
+
+// dst[i] (=|+=) alpha * sum over the patches containing DoF i of their buffered
+// contribution (additive Schwarz without atomics).
+template<int DIM, int NP, typename VT, typename CT, bool ACCUMULATE>
+__global__ void __launch_bounds__(256)
+k_avp_sum(const CT *__restrict__ ubuf, VT *__restrict__ dst, const int n, const int64_t N,
+          const double alpha)
+{
+  constexpr int NDC = cpow(NP, DIM);
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i >= N) return;
+  const int64_t cell = i / NDC;
+  const int l = int(i % NDC);
+  int c[3] = {0, 0, 0};
+  {
+    int64_t t = cell;
+    for(int d = 0; d < DIM; ++d)
+    {
+      c[d] = int(t % n);
+      t /= n;
+    }
+  }
+  CT s = 0;
+#pragma unroll
+  for(int slot = 0; slot < (1 << DIM); ++slot)
+  {
+    bool ok = true;
+#pragma unroll
+    for(int d = 0; d < DIM; ++d)
+    {
+      const int vd = c[d] + 1 - ((slot >> d) & 1);
+      ok = ok && vd >= 1 && vd <= n - 1;
+    }
+    if(ok) s += ubuf[(cell * (1 << DIM) + slot) * NDC + l];
+  }
+  if(ACCUMULATE)
+    dst[i] = VT(double(dst[i]) + alpha * double(s));
+  else
+    dst[i] = VT(alpha * double(s));
+}
+
+// ----------------------------------------------------------------------------
+// vectors
+// ----------------------------------------------------------------------------
+template<typename T>
+__global__ void k_fill(T *__restrict__ x, const int64_t n, const T value)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i < n) x[i] = value;
+}
+
+// y = a*x + b*y with host scalars
+template<typename T>
+__global__ void k_axpby(T *__restrict__ y, const T *__restrict__ x, const int64_t n,
+                        const double a, const double b)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i < n) y[i] = T(a * double(x[i]) + b * double(y[i]));
+}
+
+template<typename TD, typename TS>
+__global__ void k_convert(TD *__restrict__ dst, const TS *__restrict__ src, const int64_t n)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i < n) dst[i] = TD(src[i]);
+}
+
+inline __device__ __forceinline__ double splitmix_uniform(uint64_t seed, uint64_t i)
+{
+  uint64_t z = seed * 0x100000000ull + i;
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return 2.0 * (double(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+
+template<typename T>
+__global__ void k_random(T *__restrict__ x, const int64_t n, const uint64_t seed)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i < n) x[i] = T(splitmix_uniform(seed, uint64_t(i)));
+}
+
+// block partial sums of a.b (FP64 accumulation), deterministic two-pass reduction
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_dot_partial(const T *__restrict__ a, const T *__restrict__ b, const int64_t n,
+              double *__restrict__ partial)
+{
+  __shared__ double sh[256];
+  double s = 0.0;
+  for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+      i += int64_t(gridDim.x) * blockDim.x)
+    s += double(a[i]) * double(b[i]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for(int w = 128; w > 0; w >>= 1)
+  {
+    if(threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+static __global__ void __launch_bounds__(256)
+k_dot_final(const double *__restrict__ partial, const int np, double *__restrict__ out)
+{
+  __shared__ double sh[256];
+  double s = 0.0;
+  for(int i = threadIdx.x; i < np; i += blockDim.x) s += partial[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for(int w = 128; w > 0; w >>= 1)
+  {
+    if(threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) *out = sh[0];
+}
+
+// non-finite detector: flag[0] = 1 if any entry is NaN/Inf
+template<typename T>
+__global__ void k_nonfinite(const T *__restrict__ x, const int64_t n, int *__restrict__ flag)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i < n && !isfinite(double(x[i]))) flag[0] = 1;
+}
+
+// CG helpers with device scalars: x += (s[0]/s[1]) p ; r -= (s[0]/s[1]) Ap
+static __global__ void k_cg_update(double *__restrict__ x, double *__restrict__ r,
+                            const double *__restrict__ p, const double *__restrict__ Ap,
+                            const int64_t n, const double *__restrict__ s)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const double alpha = s[0] / s[1];
+  if(i < n)
+  {
+    x[i] += alpha * p[i];
+    r[i] -= alpha * Ap[i];
+  }
+}
+
+// p = z + (s[0]/s[1]) p
+static __global__ void k_cg_direction(double *__restrict__ p, const double *__restrict__ z,
+                               const int64_t n, const double *__restrict__ s)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const double beta = s[0] / s[1];
+  if(i < n) p[i] = z[i] + beta * p[i];
+}
+
+// ----------------------------------------------------------------------------
+// grid transfer for DG Q_p (embedding and its transpose)
+// ----------------------------------------------------------------------------
+template<int NP>
+struct TransferTab
+{
+  double P[2][NP * NP];  // P[child][i*NP + j] = phi_j((x_i + child)/2)
+};
+
+// xf += P xc ; one thread per fine DoF
+template<int DIM, int NP, typename T>
+__global__ void __launch_bounds__(256)
+k_prolongate_add(T *__restrict__ xf, const T *__restrict__ xc, const TransferTab<NP> tab,
+                 const int nc, const int64_t Nf)
+{
+  constexpr int NDC = cpow(NP, DIM);
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i >= Nf) return;
+  const int nf = 2 * nc;
+  int64_t fcell = i / NDC;
+  int l = int(i % NDC);
+  int64_t ccell = 0, st = 1;
+  int child[3], li[3];
+  for(int d = 0; d < DIM; ++d)
+  {
+    const int fc = int(fcell % nf);
+    fcell /= nf;
+    child[d] = fc & 1;
+    ccell += int64_t(fc >> 1) * st;
+    st *= nc;
+    li[d] = l % NP;
+    l /= NP;
+  }
+  const T *x = xc + ccell * NDC;
+  double s = 0.0;
+  for(int jj = 0; jj < NDC; ++jj)
+  {
+    int t = jj;
+    double w = 1.0;
+#pragma unroll
+    for(int d = 0; d < DIM; ++d)
+    {
+      w *= tab.P[child[d]][li[d] * NP + (t % NP)];
+      t /= NP;
+    }
+    s += w * double(x[jj]);
+  }
+  xf[i] = T(double(xf[i]) + s);
+}
+
+// xc = P^T xf ; one thread per coarse DoF
+template<int DIM, int NP, typename T>
+__global__ void __launch_bounds__(256)
+k_restrict(T *__restrict__ xc, const T *__restrict__ xf, const TransferTab<NP> tab, const int nc,
+           const int64_t Nc)
+{
+  constexpr int NDC = cpow(NP, DIM);
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i >= Nc) return;
+  const int nf = 2 * nc;
+  int64_t ccell = i / NDC;
+  int l = int(i % NDC);
+  int C[3], lj[3];
+  for(int d = 0; d < DIM; ++d)
+  {
+    C[d] = int(ccell % nc);
+    ccell /= nc;
+    lj[d] = l % NP;
+    l /= NP;
+  }
+  double s = 0.0;
+  for(int child = 0; child < (1 << DIM); ++child)
+  {
+    int64_t fcell = 0, st = 1;
+    for(int d = 0; d < DIM; ++d)
+    {
+      fcell += int64_t(2 * C[d] + ((child >> d) & 1)) * st;
+      st *= nf;
+    }
+    const T *x = xf + fcell * NDC;
+    for(int ii = 0; ii < NDC; ++ii)
+    {
+      int t = ii;
+      double w = 1.0;
+#pragma unroll
+      for(int d = 0; d < DIM; ++d)
+      {
+        w *= tab.P[(child >> d) & 1][(t % NP) * NP + lj[d]];
+        t /= NP;
+      }
+      s += w * double(x[ii]);
+    }
+  }
+  xc[i] = T(s);
+}
+
+// y = A x with a dense row-major FP64 matrix (coarse-grid inverse); one
+// 64-lane wave per row
+static __global__ void __launch_bounds__(256)
+k_dense_matvec(const double *__restrict__ A, const double *__restrict__ x, double *__restrict__ y,
+               const int m)
+{
+  const int row = int(blockIdx.x) * 4 + int(threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if(row >= m) return;
+  double s = 0.0;
+  for(int c = lane; c < m; c += 64) s += A[size_t(row) * m + c] * x[c];
+  for(int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if(lane == 0) y[row] = s;
+}
+
+}  // namespace tpss

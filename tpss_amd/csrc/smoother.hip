@@ -1,0 +1,657 @@
+// smoother.hip -- SchwarzPreconditioner / SchwarzSmoother on the device.
+//
+// Setup (SubdomainHandler::reinit + compute_inverses): patch gathering and
+// coloring on the host (index arithmetic only), surrogate lengths and FP64
+// local-solver factors (1D generalized eigenproblems) on the device.
+// Apply: fused gather -> fast-diagonalization -> damping -> scatter kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <tuple>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "ops.hpp"
+#include "setup_kernels.hpp"
+#include "smoother.hpp"
+
+namespace tpss
+{
+namespace
+{
+// RedBlack / Tiled vertex-patch colors (apps/coloring.h:286-324, 359-390)
+int patch_color(int dim, int coloring, const int *v)
+{
+  if(coloring == TPSS_COLOR_NONE) return 0;
+  int mask = 0, sum = 0;
+  for(int d = 0; d < dim; ++d)
+  {
+    const int x = v[d] - 1, s = x & 1;
+    mask |= s << d;
+    sum += (x + s) / 2;
+  }
+  if(coloring == TPSS_COLOR_TILED) return mask;
+  return 2 * mask + (sum & 1);
+}
+
+bool patch_at_boundary(int dim, int n, const int *v)
+{
+  for(int d = 0; d < dim; ++d)
+    if(v[d] == 1 || v[d] == n - 1) return true;
+  return false;
+}
+
+int n_colors_of(int dim, int coloring)
+{
+  return coloring == TPSS_COLOR_NONE ? 1 : coloring == TPSS_COLOR_TILED ? (1 << dim) : (2 << dim);
+}
+
+template<int NP>
+Shape1D<NP> make_shape(int p)
+{
+  const Basis1D b = make_basis(p);
+  Shape1D<NP> s;
+  for(int e = 0; e < NP * NP; ++e)
+  {
+    s.val[e] = b.val[e];
+    s.grad[e] = b.grad[e];
+  }
+  for(int e = 0; e < NP * 2; ++e)
+  {
+    s.fval[e] = b.fval[e];
+    s.fgrad[e] = b.fgrad[e];
+  }
+  for(int q = 0; q < NP; ++q) s.qw[q] = b.qw[q];
+  return s;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------- setup
+
+std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int coloring,
+                                                  hipStream_t s)
+{
+  auto t0 = std::chrono::steady_clock::now();
+  auto ls = std::make_shared<LocalSolvers>();
+  const int dim = lv->dim, n = lv->n;
+  ls->dim = dim;
+  ls->p = lv->p;
+  ls->np = lv->np;
+  ls->K = 2 * lv->np;
+  ls->n = n;
+  ls->coloring = coloring;
+  ls->n_colors = n_colors_of(dim, coloring);
+  const int P = ipow(n - 1, dim);
+  ls->P = P;
+
+  // patch order: color, interior before boundary (patch_info.templates.h:611-645),
+  // vertex lexicographic inside each group
+  ls->color_ptr.assign(ls->n_colors + 1, 0);
+  ls->vtx.resize(P);
+  ls->vtx_unpacked.resize(size_t(P) * dim);
+  int pos = 0;
+  for(int col = 0; col < ls->n_colors; ++col)
+  {
+    ls->color_ptr[col] = pos;
+    for(int pass = 0; pass < 2; ++pass)
+      for(int lin = 0; lin < P; ++lin)
+      {
+        int v[3] = {1, 1, 1}, t = lin;
+        for(int d = 0; d < dim; ++d)
+        {
+          v[d] = 1 + t % (n - 1);
+          t /= (n - 1);
+        }
+        if(patch_color(dim, coloring, v) != col) continue;
+        if(int(patch_at_boundary(dim, n, v)) != pass) continue;
+        ls->vtx[pos] = uint32_t(v[0]) | (uint32_t(v[1]) << 10) | (uint32_t(v[2]) << 20);
+        for(int d = 0; d < dim; ++d) ls->vtx_unpacked[size_t(pos) * dim + d] = v[d];
+        ++pos;
+      }
+  }
+  ls->color_ptr[ls->n_colors] = pos;
+  ls->d_vtx.upload(ls->vtx);
+
+  // 1D problems: (h_left, h_right, boundary bits) per patch and direction
+  std::vector<double> hs(size_t(P) * dim * 2);
+  std::vector<uint8_t> bd(size_t(P) * dim * 2);
+  if(lv->distorted)
+  {
+    DeviceArray<double> d_h(hs.size());
+    DeviceArray<uint8_t> d_b(bd.size());
+    const int nq = std::min(8, std::max(2, lv->p + 1));  // apps/poisson_distorted.cc:88
+    double gx[8], gw[8];
+    gauss_lobatto_nodes(nq, gx, gw);
+    DeviceArray<double> d_gx, d_gw;
+    d_gx.upload(std::vector<double>(gx, gx + nq));
+    d_gw.upload(std::vector<double>(gw, gw + nq));
+    auto launch = [&](auto D, auto NQ) {
+      constexpr int DIM = decltype(D)::value, Q = decltype(NQ)::value;
+      hipLaunchKernelGGL((k_surrogate_h<DIM, Q>), dim3(blocks_for(P, 128)), dim3(128), 0, s,
+                         lv->d_vertices.ptr, ls->d_vtx.ptr, P, n, d_gx.ptr, d_gw.ptr, d_h.ptr,
+                         d_b.ptr);
+    };
+    if(dim == 2)
+    {
+      if(nq == 2) launch(ic<2>{}, ic<2>{});
+      else if(nq == 3) launch(ic<2>{}, ic<3>{});
+      else if(nq == 4) launch(ic<2>{}, ic<4>{});
+      else launch(ic<2>{}, ic<5>{});
+    }
+    else
+    {
+      if(nq == 2) launch(ic<3>{}, ic<2>{});
+      else if(nq == 3) launch(ic<3>{}, ic<3>{});
+      else if(nq == 4) launch(ic<3>{}, ic<4>{});
+      else launch(ic<3>{}, ic<5>{});
+    }
+    TPSS_HIP(hipGetLastError());
+    TPSS_HIP(hipStreamSynchronize(s));
+    hs = d_h.download();
+    bd = d_b.download();
+    ls->nfac = P * dim;
+    ls->fac.resize(size_t(P) * dim);
+    for(size_t i = 0; i < ls->fac.size(); ++i) ls->fac[i] = uint32_t(i);
+    ls->prob_h = hs;
+    ls->prob_b = bd;
+  }
+  else
+  {
+    for(int jp = 0; jp < P; ++jp)
+      for(int d = 0; d < dim; ++d)
+      {
+        const int vd = ls->vtx_unpacked[size_t(jp) * dim + d];
+        hs[(size_t(jp) * dim + d) * 2 + 0] = lv->h;
+        hs[(size_t(jp) * dim + d) * 2 + 1] = lv->h;
+        bd[(size_t(jp) * dim + d) * 2 + 0] = vd == 1;
+        bd[(size_t(jp) * dim + d) * 2 + 1] = vd == n - 1;
+      }
+    // deduplicate identical 1D problems
+    std::map<std::tuple<double, double, int, int>, uint32_t> uniq;
+    ls->fac.resize(size_t(P) * dim);
+    for(size_t i = 0; i < ls->fac.size(); ++i)
+    {
+      auto key = std::make_tuple(hs[2 * i], hs[2 * i + 1], int(bd[2 * i]), int(bd[2 * i + 1]));
+      auto it = uniq.find(key);
+      if(it == uniq.end())
+      {
+        const uint32_t id = uint32_t(uniq.size());
+        uniq.emplace(key, id);
+        ls->prob_h.push_back(hs[2 * i]);
+        ls->prob_h.push_back(hs[2 * i + 1]);
+        ls->prob_b.push_back(bd[2 * i]);
+        ls->prob_b.push_back(bd[2 * i + 1]);
+        ls->fac[i] = id;
+      }
+      else
+        ls->fac[i] = it->second;
+    }
+    ls->nfac = int(uniq.size());
+  }
+  ls->h_host = hs;
+  ls->d_fac.upload(ls->fac);
+
+  // FP64 factors on the device
+  const int K = ls->K, nf = ls->nfac;
+  ls->S64.resize(size_t(nf) * K * K);
+  ls->lam64.resize(size_t(nf) * K);
+  {
+    DeviceArray<double> d_h;
+    DeviceArray<uint8_t> d_b;
+    d_h.upload(ls->prob_h);
+    d_b.upload(ls->prob_b);
+    DeviceArray<int> d_status(1);
+    TPSS_HIP(hipMemsetAsync(d_status.ptr, 0, sizeof(int), s));
+    dispatch_dim_np(dim, lv->np, [&](auto, auto NPc) {
+      constexpr int NP = decltype(NPc)::value;
+      const auto sh = make_shape<NP>(NP - 1);
+      hipLaunchKernelGGL((k_setup_fd<NP>), dim3(blocks_for(nf, 64)), dim3(64), 0, s, d_h.ptr,
+                         d_b.ptr, sh, NP - 1, lv->ip, nf, ls->S64.ptr, ls->lam64.ptr, d_status.ptr);
+      TPSS_HIP(hipGetLastError());
+    });
+    int status = 0;
+    TPSS_HIP(hipMemcpyAsync(&status, d_status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    TPSS_HIP(hipStreamSynchronize(s));
+    if(status) throw Error(TPSS_E_SINGULAR, "local solver setup: mass matrix not s.p.d. or zero eigenvalue");
+  }
+  ls->S32.resize(ls->S64.n);
+  ls->lam32.resize(ls->lam64.n);
+  convert(ls->S32.ptr, TPSS_F32, ls->S64.ptr, TPSS_F64, int64_t(ls->S64.n), s);
+  convert(ls->lam32.ptr, TPSS_F32, ls->lam64.ptr, TPSS_F64, int64_t(ls->lam64.n), s);
+
+  // cells covered by each color (targets of the localized residual update)
+  ls->color_cells.resize(ls->n_colors);
+  ls->color_ncells.assign(ls->n_colors, 0);
+  for(int col = 0; col < ls->n_colors; ++col)
+  {
+    std::vector<uint32_t> cells;
+    for(int jp = ls->color_ptr[col]; jp < ls->color_ptr[col + 1]; ++jp)
+      for(int cb = 0; cb < (1 << dim); ++cb)
+      {
+        int64_t c = 0, st = 1;
+        for(int d = 0; d < dim; ++d)
+        {
+          c += int64_t(ls->vtx_unpacked[size_t(jp) * dim + d] - 1 + ((cb >> d) & 1)) * st;
+          st *= n;
+        }
+        cells.push_back(uint32_t(c));
+      }
+    std::sort(cells.begin(), cells.end());
+    cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+    ls->color_ncells[col] = int64_t(cells.size());
+    ls->color_cells[col] = std::make_shared<DeviceArray<uint32_t>>();
+    ls->color_cells[col]->upload(cells);
+  }
+  TPSS_HIP(hipStreamSynchronize(s));
+  ls->setup_seconds =
+    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return ls;
+}
+
+// ---------------------------------------------------------------------------- apply
+
+template<typename VT, typename CT>
+void patch_pass(tpss_smoother *sm, const VT *src, VT *dst, CT *ubuf, int begin, int end, int mode,
+                hipStream_t s)
+{
+  const LocalSolvers &ls = *sm->ls;
+  if(end <= begin) return;
+  const CT *S = std::is_same<CT, double>::value ? (const CT *)ls.S64.ptr : (const CT *)ls.S32.ptr;
+  const CT *lam =
+    std::is_same<CT, double>::value ? (const CT *)ls.lam64.ptr : (const CT *)ls.lam32.ptr;
+  const float *om = sm->use_omega ? sm->d_omega.ptr : nullptr;
+  const CT relax = CT(sm->desc.relaxation);
+  dispatch_dim_np(ls.dim, ls.np, [&](auto D, auto NPc) {
+    constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value, K = 2 * NP;
+    using PS = PatchShape<DIM, K>;
+    const unsigned grid = unsigned((end - begin + PS::PPB - 1) / PS::PPB);
+    const unsigned threads = unsigned(PS::THREADS);
+    if(mode == 0)
+      hipLaunchKernelGGL((k_patch_fd<DIM, K, VT, CT, 0>), dim3(grid), dim3(threads), 0, s, src,
+                         dst, ubuf, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end, ls.n);
+    else
+      hipLaunchKernelGGL((k_patch_fd<DIM, K, VT, CT, 1>), dim3(grid), dim3(threads), 0, s, src,
+                         dst, ubuf, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end, ls.n);
+    TPSS_HIP(hipGetLastError());
+  });
+}
+
+template<typename VT, typename CT>
+void avp_sum(tpss_smoother *sm, const CT *ubuf, VT *dst, bool accumulate, double alpha,
+             hipStream_t s)
+{
+  const tpss_level *lv = sm->level;
+  dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
+    constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
+    const unsigned g = blocks_for(lv->N, 256);
+    if(accumulate)
+      hipLaunchKernelGGL((k_avp_sum<DIM, NP, VT, CT, true>), dim3(g), dim3(256), 0, s, ubuf, dst,
+                         lv->n, lv->N, alpha);
+    else
+      hipLaunchKernelGGL((k_avp_sum<DIM, NP, VT, CT, false>), dim3(g), dim3(256), 0, s, ubuf,
+                         dst, lv->n, lv->N, alpha);
+    TPSS_HIP(hipGetLastError());
+  });
+}
+
+struct SectionTimer
+{
+  tpss_smoother *sm;
+  int idx;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  SectionTimer(tpss_smoother *sm_, int idx_, hipStream_t s_) : sm(sm_), idx(idx_), s(s_)
+  {
+    if(!sm->timing) return;
+    TPSS_HIP(hipEventCreate(&a));
+    TPSS_HIP(hipEventCreate(&b));
+    TPSS_HIP(hipEventRecord(a, s));
+  }
+  ~SectionTimer()
+  {
+    if(!sm->timing) return;
+    float ms = 0.f;
+    if(hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+       hipEventElapsedTime(&ms, a, b) == hipSuccess)
+    {
+      sm->timers[idx].time += 1e-3 * ms;
+      sm->timers[idx].n_calls += 1;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+
+// dst = P src  (or, if accumulate: dst += alpha * P src for the additive variant)
+template<typename VT, typename CT>
+void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
+                     double alpha)
+{
+  LocalSolvers &ls = *sm->ls;
+  const tpss_level *lv = sm->level;
+  SectionTimer step_timer(sm, 4, s);
+  if(sm->desc.variant == TPSS_ADDITIVE)
+  {
+    sm->ubuf.resize(size_t(lv->N) * (1 << lv->dim) * sizeof(CT));
+    CT *ub = (CT *)sm->ubuf.ptr;
+    {
+      SectionTimer t(sm, 1, s);
+      patch_pass<VT, CT>(sm, src, nullptr, ub, 0, ls.P, 1, s);
+      avp_sum<VT, CT>(sm, ub, dst, accumulate, accumulate ? alpha : 1.0, s);
+    }
+    return;
+  }
+  // multiplicative: colors in sequence, localized residual before each color
+  VT *out = dst;
+  if(accumulate)
+  {
+    sm->tmp2.resize(size_t(lv->N) * sizeof(VT));
+    out = (VT *)sm->tmp2.ptr;
+  }
+  sm->res.resize(size_t(lv->N) * sizeof(VT));
+  VT *res = (VT *)sm->res.ptr;
+  fill_zero(out, lv->vec_prec, lv->N, s);
+  for(size_t k = 0; k < sm->seq.size(); ++k)
+  {
+    const int c = sm->seq[k];
+    const VT *r = src;
+    if(k > 0)
+    {
+      SectionTimer t(sm, 0, s);
+      operator_apply_any(lv, res, out, src, ls.color_cells[c]->ptr, ls.color_ncells[c], s);
+      r = res;
+    }
+    SectionTimer t(sm, 1, s);
+    patch_pass<VT, CT>(sm, r, out, nullptr, ls.color_ptr[c], ls.color_ptr[c + 1], 0, s);
+  }
+  if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
+}
+
+template<typename VT>
+void precond_dispatch(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
+                      double alpha)
+{
+  const int lp = sm->desc.local_precision;
+  if(std::is_same<VT, double>::value && lp == TPSS_F64)
+    precond_vmult_t<VT, double>(sm, dst, src, s, accumulate, alpha);
+  else if(lp == TPSS_F32)
+    precond_vmult_t<VT, float>(sm, dst, src, s, accumulate, alpha);
+  else if(lp == TPSS_BF16)
+    precond_vmult_bf16(sm, dst, src, s, accumulate, alpha);
+  else
+    throw Error(TPSS_E_UNSUPPORTED, "local precision F64 needs F64 level vectors");
+}
+
+void precond_vmult_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
+                       bool accumulate, double alpha)
+{
+  if(sm->level->vec_prec == TPSS_F64)
+    precond_dispatch<double>(sm, (double *)dst, (const double *)src, s, accumulate, alpha);
+  else
+    precond_dispatch<float>(sm, (float *)dst, (const float *)src, s, accumulate, alpha);
+}
+
+// SmootherRichardson::step_impl: x <- x + omega_R P (b - A x), n_steps times
+void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s)
+{
+  const tpss_level *lv = sm->level;
+  const size_t bytes = size_t(lv->N) * elem_size(lv->vec_prec);
+  sm->rich_r.resize(bytes);
+  for(int k = 0; k < sm->desc.n_steps; ++k)
+  {
+    operator_apply_any(lv, sm->rich_r.ptr, dst, src, nullptr, 0, s);
+    precond_vmult_any(sm, dst, sm->rich_r.ptr, s, true, sm->desc.damping_factor);
+  }
+}
+
+void check_finite(tpss_smoother *sm, const void *x, hipStream_t s)
+{
+  if(!sm->check_finite) return;
+  const tpss_level *lv = sm->level;
+  sm->flag.resize(1);
+  TPSS_HIP(hipMemsetAsync(sm->flag.ptr, 0, sizeof(int), s));
+  if(lv->vec_prec == TPSS_F64)
+    hipLaunchKernelGGL((k_nonfinite<double>), dim3(blocks_for(lv->N, 256)), dim3(256), 0, s,
+                       (const double *)x, lv->N, sm->flag.ptr);
+  else
+    hipLaunchKernelGGL((k_nonfinite<float>), dim3(blocks_for(lv->N, 256)), dim3(256), 0, s,
+                       (const float *)x, lv->N, sm->flag.ptr);
+  TPSS_HIP(hipGetLastError());
+  int f = 0;
+  TPSS_HIP(hipMemcpyAsync(&f, sm->flag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  TPSS_HIP(hipStreamSynchronize(s));
+  if(f) throw Error(TPSS_E_NONFINITE, "non-finite entries in the smoother output");
+}
+
+std::vector<int> color_sequence(int n_colors, bool reverse, bool symmetrized)
+{
+  // get_color_sequence, schwarz_preconditioner.h:531-567
+  std::vector<int> seq(n_colors);
+  for(int i = 0; i < n_colors; ++i) seq[i] = i;
+  if(reverse) std::reverse(seq.begin(), seq.end());
+  if(symmetrized)
+    for(int i = n_colors - 2; i >= 0; --i) seq.push_back(seq[i]);
+  return seq;
+}
+
+void validate_desc(const tpss_smoother_desc *d)
+{
+  TPSS_REQUIRE(d, "null smoother descriptor");
+  TPSS_REQUIRE(d->variant == TPSS_ADDITIVE || d->variant == TPSS_MULTIPLICATIVE,
+               "invalid smoother variant");
+  TPSS_REQUIRE(d->coloring >= TPSS_COLOR_NONE && d->coloring <= TPSS_COLOR_REDBLACK,
+               "invalid coloring");
+  TPSS_REQUIRE(d->relaxation > 0.0, "relaxation must be positive");
+  TPSS_REQUIRE(d->n_steps >= 1, "n_steps must be >= 1");
+  TPSS_REQUIRE(d->local_precision >= TPSS_F64 && d->local_precision <= TPSS_BF16,
+               "invalid local precision");
+  TPSS_REQUIRE(!(d->variant == TPSS_MULTIPLICATIVE && d->coloring == TPSS_COLOR_NONE),
+               "the multiplicative variant needs a coloring (RedBlack for DG)");
+}
+
+}  // namespace tpss
+
+using namespace tpss;
+
+extern "C" {
+
+int tpss_smoother_create(tpss_level *level, const tpss_smoother_desc *desc, tpss_smoother **out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(level && out, "null argument");
+    validate_desc(desc);
+    if(level->vec_prec == TPSS_F32 && desc->local_precision == TPSS_F64)
+      throw Error(TPSS_E_UNSUPPORTED, "local precision F64 needs F64 level vectors");
+    auto sm = std::make_unique<tpss_smoother>();
+    sm->level = level;
+    sm->desc = *desc;
+    sm->ls = build_local_solvers(level, desc->coloring, nullptr);
+    sm->seq = color_sequence(sm->ls->n_colors, desc->reverse != 0, desc->symmetrized != 0);
+    sm->timers[2].time = sm->ls->setup_seconds;
+    sm->timers[2].n_calls = 1;
+    *out = sm.release();
+  });
+}
+
+int tpss_smoother_clone_shallow(const tpss_smoother *other, const tpss_smoother_desc *desc,
+                                tpss_smoother **out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(other && out, "null argument");
+    validate_desc(desc);
+    TPSS_REQUIRE(desc->coloring == other->desc.coloring,
+                 "a shallow copy shares the patches and must keep the coloring");
+    auto sm = std::make_unique<tpss_smoother>();
+    sm->level = other->level;
+    sm->desc = *desc;
+    sm->ls = other->ls;
+    sm->seq = color_sequence(sm->ls->n_colors, desc->reverse != 0, desc->symmetrized != 0);
+    if(other->use_omega)
+    {
+      sm->d_omega.resize(other->d_omega.n);
+      TPSS_HIP(hipMemcpy(sm->d_omega.ptr, other->d_omega.ptr, other->d_omega.n * sizeof(float),
+                         hipMemcpyDeviceToDevice));
+      sm->use_omega = true;
+    }
+    *out = sm.release();
+  });
+}
+
+int tpss_smoother_destroy(tpss_smoother *sm)
+{
+  delete sm;
+  return TPSS_OK;
+}
+
+int tpss_precond_vmult(tpss_smoother *sm, void *dst, const void *src, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && dst && src, "null argument");
+    TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    precond_vmult_any(sm, dst, src, as_stream(stream), false, 1.0);
+    check_finite(sm, dst, as_stream(stream));
+  });
+}
+
+int tpss_smoother_step(tpss_smoother *sm, void *dst, const void *src, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && dst && src, "null argument");
+    TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    smoother_step_any(sm, dst, src, as_stream(stream));
+    check_finite(sm, dst, as_stream(stream));
+  });
+}
+
+int tpss_smoother_vmult(tpss_smoother *sm, void *dst, const void *src, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && dst && src, "null argument");
+    TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    fill_zero(dst, sm->level->vec_prec, sm->level->N, as_stream(stream));
+    smoother_step_any(sm, dst, src, as_stream(stream));
+    check_finite(sm, dst, as_stream(stream));
+  });
+}
+
+int tpss_smoother_set_patch_damping(tpss_smoother *sm, const double *omega)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm, "null argument");
+    if(!omega)
+    {
+      sm->use_omega = false;
+      return;
+    }
+    std::vector<float> h(sm->ls->P);
+    for(int i = 0; i < sm->ls->P; ++i)
+    {
+      TPSS_REQUIRE(omega[i] > 0.0 && std::isfinite(omega[i]), "patch damping must be positive");
+      h[i] = float(omega[i]);
+    }
+    sm->d_omega.upload(h);
+    sm->use_omega = true;
+  });
+}
+
+int tpss_smoother_info(const tpss_smoother *sm, int *n_patches, int *n_colors, int *n_dofs_per_patch)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm, "null argument");
+    if(n_patches) *n_patches = sm->ls->P;
+    if(n_colors) *n_colors = sm->ls->n_colors;
+    if(n_dofs_per_patch) *n_dofs_per_patch = ipow(sm->ls->K, sm->ls->dim);
+  });
+}
+
+int tpss_smoother_color_ptr(const tpss_smoother *sm, int *out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && out, "null argument");
+    std::copy(sm->ls->color_ptr.begin(), sm->ls->color_ptr.end(), out);
+  });
+}
+
+int tpss_smoother_patch_vertices(const tpss_smoother *sm, int *out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && out, "null argument");
+    std::copy(sm->ls->vtx_unpacked.begin(), sm->ls->vtx_unpacked.end(), out);
+  });
+}
+
+int tpss_smoother_patch_dof_indices(const tpss_smoother *sm, uint32_t *out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && out, "null argument");
+    patch_dof_indices_device(*sm->ls, out);
+  });
+}
+
+int tpss_smoother_patch_h(const tpss_smoother *sm, double *out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && out, "null argument");
+    std::copy(sm->ls->h_host.begin(), sm->ls->h_host.end(), out);
+  });
+}
+
+int tpss_smoother_get_timings(const tpss_smoother *sm, tpss_time_info *out, int *n)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && n, "null argument");
+    static const char *names[5] = {"[SchwarzPrecond] Update residuals:",
+                                   "[SchwarzPrecond] Apply inverses:",
+                                   "[SchwarzPrecond] Compute inverses:",
+                                   "[SchwarzPrecond] Copy Locally Owned:",
+                                   "[SchwarzPrecond] Step:"};
+    const int cap = *n;
+    *n = 5;
+    if(!out) return;
+    for(int i = 0; i < std::min(cap, 5); ++i)
+    {
+      out[i].time = sm->timers[i].time;
+      out[i].description = names[i];
+      out[i].n_calls = sm->timers[i].n_calls;
+    }
+  });
+}
+
+int tpss_smoother_set_timing(tpss_smoother *sm, int enabled)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm, "null argument");
+    sm->timing = enabled != 0;
+  });
+}
+
+int tpss_smoother_set_check_finite(tpss_smoother *sm, int enabled)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm, "null argument");
+    sm->check_finite = enabled != 0;
+  });
+}
+
+}  // extern "C"
+
+namespace tpss
+{
+void patch_dof_indices_device(const LocalSolvers &ls, uint32_t *host_out)
+{
+  const int64_t total = int64_t(ls.P) * ipow(ls.K, ls.dim);
+  DeviceArray<uint32_t> d_out{size_t(total)};
+  dispatch_dim_np(ls.dim, ls.np, [&](auto D, auto NPc) {
+    constexpr int DIM = decltype(D)::value, K = 2 * decltype(NPc)::value;
+    hipLaunchKernelGGL((k_patch_dof_indices<DIM, K>), dim3(blocks_for(total, 256)), dim3(256), 0,
+                       nullptr, ls.d_vtx.ptr, ls.P, ls.n, d_out.ptr);
+    TPSS_HIP(hipGetLastError());
+  });
+  TPSS_HIP(hipMemcpy(host_out, d_out.ptr, size_t(total) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+}
+}  // namespace tpss

@@ -1,0 +1,64 @@
+// smoother.hpp -- internal state of the Schwarz smoother shared by the
+// translation units of libtpss_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+
+namespace tpss
+{
+// Patches, colors and FP64/FP32 local-solver factors of one level; shared
+// between shallow copies (SchwarzPreconditioner::initialize(other, data)).
+struct LocalSolvers
+{
+  int dim = 0, p = 0, np = 0, K = 0, n = 0, P = 0, n_colors = 0, coloring = 0;
+  std::vector<int> color_ptr;
+  std::vector<uint32_t> vtx;       // packed vertex per patch
+  std::vector<int> vtx_unpacked;   // P * dim
+  DeviceArray<uint32_t> d_vtx;
+  std::vector<uint32_t> fac;       // P * dim rows of the factor table
+  DeviceArray<uint32_t> d_fac;
+  int nfac = 0;
+  std::vector<double> prob_h;      // nfac * 2
+  std::vector<uint8_t> prob_b;     // nfac * 2
+  DeviceArray<double> S64, lam64;  // nfac * K * K, nfac * K
+  DeviceArray<float> S32, lam32;
+  std::vector<double> h_host;      // P * dim * 2 surrogate lengths
+  std::vector<std::shared_ptr<DeviceArray<uint32_t>>> color_cells;
+  std::vector<int64_t> color_ncells;
+  double setup_seconds = 0.0;
+};
+
+void patch_dof_indices_device(const LocalSolvers &ls, uint32_t *host_out);
+
+}  // namespace tpss
+
+struct tpss_smoother
+{
+  tpss_level *level = nullptr;
+  tpss_smoother_desc desc{};
+  std::shared_ptr<tpss::LocalSolvers> ls;
+  std::vector<int> seq;
+  tpss::DeviceArray<float> d_omega;
+  bool use_omega = false;
+  bool timing = false;
+  bool check_finite = false;
+  tpss::Timer timers[5];
+  // scratch (bytes)
+  tpss::DeviceArray<char> res, tmp2, rich_r, ubuf;
+  tpss::DeviceArray<int> flag;
+};
+
+namespace tpss
+{
+void precond_vmult_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
+                       bool accumulate, double alpha);
+void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s);
+template<typename VT>
+void precond_vmult_bf16(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
+                        double alpha);
+}  // namespace tpss
