@@ -1,0 +1,196 @@
+"""Benchmark of the vertex-patch Schwarz smoother hot path on one MI355X.
+
+Workload: 3D Poisson, SIPG DG Q3 (64 DoFs/cell) on 64^3 Cartesian cells
+(16.8M DoFs), multiplicative vertex-patch smoother (2x2x2-cell patches, 512
+DoFs/patch, 16 red-black colors), seeded U[-1,1] right-hand side (synthetic).
+
+A "step" is one SchwarzSmoother::step (Richardson: residual + one full
+multiplicative Schwarz sweep over all patches), inputs resident in HBM.
+value = patch-DoFs/s of the mixed-precision smoother (FP32 vectors and patch
+solves; the FP64 smoother is reported beside it), plus the GMG-preconditioned
+CG time to a 1e-8 relative residual for the FP64 and the mixed V-cycle
+(outer CG, residual and coarse solve in FP64).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="cells per direction")
+    ap.add_argument("--degree", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-gmg", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+def residual_kernel_bytes(sm, n, ndc, s_v):
+    """Algorithmic bytes of the localized residual r = b - A x on the cells of
+    each color (read x on the cells and their face neighbours, read b, write r),
+    averaged over the colors that get a residual update."""
+    V = sm.patch_vertices()
+    cp = sm.color_ptr()
+    dim = V.shape[1]
+    totals = []
+    for c in range(1, sm.n_colors):  # colors after the first need a residual
+        vs = V[cp[c]:cp[c + 1]]
+        cells = set()
+        for off in np.ndindex(*([2] * dim)):
+            cc = vs - 1 + np.array(off)
+            lin = cc[:, 0] + n * cc[:, 1] + (n * n * cc[:, 2] if dim == 3 else 0)
+            cells.update(lin.tolist())
+        cells = np.array(sorted(cells))
+        coords = np.stack([(cells // n ** d) % n for d in range(dim)], axis=1)
+        touched = set(cells.tolist())
+        for d in range(dim):
+            for s in (-1, 1):
+                nb = coords.copy()
+                nb[:, d] += s
+                ok = (nb[:, d] >= 0) & (nb[:, d] < n)
+                lin = sum(nb[ok, e] * n ** e for e in range(dim))
+                touched.update(lin.tolist())
+        totals.append((2 * len(cells) + len(touched)) * ndc * s_v)
+    return float(np.mean(totals))
+
+
+def time_smoother(t, lv, sm, steps, warmup, precision):
+    b = t.DeviceVector(lv.n_dofs, precision)
+    t.fill_random(b, 2)
+    x = t.DeviceVector.zeros(lv.n_dofs, precision)
+    for _ in range(warmup):
+        sm.step(x, b)
+    t.synchronize()
+    sm.set_timing(True)
+    t.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sm.step(x, b)
+    t.synchronize()
+    t1 = time.perf_counter()
+    sections = {name: (secs, calls) for name, secs, calls in sm.get_time_data()}
+    sm.set_timing(False)
+    return (t1 - t0) / steps, sections
+
+
+def cpu_baseline(n, p, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as o
+    sm = o.Smoother(3, p, n, variant=o.MULTIPLICATIVE, coloring=o.COLOR_REDBLACK, threads=threads)
+    b = o.splitmix_uniform(2, sm.N)
+    x = np.zeros(sm.N)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sm.step(x, b)
+    dt = (time.perf_counter() - t0) / steps
+    return sm.P * sm.ndp / dt, dt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the smoother path is single-GPU; see the sharding design note).", file=sys.stderr)
+        sys.exit(2)
+    import tpss_amd as t
+
+    n, p, dim = args.n, args.degree, 3
+    ndc = (p + 1) ** dim
+    results = {}
+    for name, prec in (("f64", t.F64), ("f32", t.F32)):
+        lv = t.Level(dim, p, n, precision=prec)
+        sm = t.SchwarzSmoother(lv, variant=t.MULTIPLICATIVE, coloring=t.COLOR_REDBLACK,
+                               relaxation=1.0, local_precision=prec)
+        sec, sections = time_smoother(t, lv, sm, args.steps, args.warmup, prec)
+        s_v = 8 if prec == t.F64 else 4
+        pd = sm.n_patches * sm.n_dofs_per_patch
+        apply_t, apply_n = sections["[SchwarzPrecond] Apply inverses:"]
+        res_t, res_n = sections["[SchwarzPrecond] Update residuals:"]
+        patch_bytes = sm.n_patches / sm.n_colors * 3 * sm.n_dofs_per_patch * s_v
+        res_bytes = residual_kernel_bytes(sm, n, ndc, s_v)
+        kernels = {
+            "patch_fd": {"avg_ms": 1e3 * apply_t / max(apply_n, 1), "launches": apply_n,
+                         "bytes": patch_bytes},
+            "residual": {"avg_ms": 1e3 * res_t / max(res_n, 1), "launches": res_n,
+                         "bytes": res_bytes},
+        }
+        for k in kernels.values():
+            k["gbs"] = k["bytes"] / (k["avg_ms"] * 1e-3) / 1e9 if k["avg_ms"] > 0 else 0.0
+        dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+        results[name] = {"ms_per_step": 1e3 * sec, "patch_dofs_per_s": pd / sec,
+                         "kernels": kernels, "dominant": dom, "n_patches": sm.n_patches,
+                         "n_dofs": lv.n_dofs}
+        del sm, lv
+
+    gmg = {}
+    if not args.no_gmg:
+        for name, mp in (("f64", t.F64), ("mixed_f32", t.F32)):
+            g = t.GMGSolver(dim, p, 2, int(np.log2(n // 2)) + 1, mg_precision=mp,
+                            variant=t.MULTIPLICATIVE, coloring=t.COLOR_REDBLACK, relaxation=1.0)
+            b = t.DeviceVector(g.n_dofs)
+            t.fill_random(b, 2)
+            x = t.DeviceVector.zeros(g.n_dofs)
+            g.solve(x, b, rel_tol=1e-8)  # warm-up solve
+            info, hist = g.solve(x, b, rel_tol=1e-8)
+            gmg[name] = {"seconds": info.seconds, "iterations": info.iterations,
+                         "n_frac": info.n_frac, "reduction_rate": info.reduction_rate,
+                         "final_rel_residual": hist[-1] / hist[0]}
+            del g
+
+    cpu = None
+    if not args.no_cpu:
+        rate, dt = cpu_baseline(n, p, args.cpu_steps, args.cpu_threads)
+        cpu = {"value": rate, "unit": "patch-DoFs/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"{args.cpu_steps} FP64 MVP smoothing steps of the C oracle on the same "
+                         f"{n}^3 Q{p} level ({dt:.2f} s/step, full residual per color as the "
+                         f"reference)"}
+
+    main_v = results["f32"]
+    dom = main_v["dominant"]
+    k = main_v["kernels"][dom]
+    out = {
+        "metric": "smoother patch-DoFs/s + GMG-CG time-to-1e-8: mixed/learned vs FP64, 1 MI355X",
+        "value": main_v["patch_dofs_per_s"],
+        "unit": "patch-DoFs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_v["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded U[-1,1] right-hand side, Cartesian unit cube)",
+        "config": {"workload": f"3D SIPG DG Q{p} Poisson {n}^3 cells, multiplicative vertex-patch "
+                               f"Schwarz smoother step (16 red-black colors, 512 DoFs/patch)",
+                   "n_dofs": main_v["n_dofs"], "n_patches": main_v["n_patches"],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": k["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": k["gbs"] / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "variants": {name: {"patch_dofs_per_s": r["patch_dofs_per_s"],
+                            "ms_per_step": r["ms_per_step"],
+                            "kernels": r["kernels"]} for name, r in results.items()},
+        "gmg_cg": gmg,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,9 @@ def _load():
         "or_gmg_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_int, ctypes.c_double]),
+        "or_gmg_create_v": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]),
         "or_gmg_destroy": (None, [ctypes.c_void_p]),
         "or_gmg_level": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int]),
         "or_gmg_vcycle": (None, [ctypes.c_void_p, _D, _D]),
@@ -275,9 +278,14 @@ class GMG:
     """CPU FP64 geometric multigrid + CG (oracle)."""
 
     def __init__(self, dim, p, n_coarse, n_levels, ip=1.0, variant=MULTIPLICATIVE,
-                 coloring=COLOR_REDBLACK, relaxation=1.0, n_steps=1, coarse_tol=1e-12):
-        self.h = lib.or_gmg_create(dim, p, n_coarse, n_levels, float(ip), variant, coloring,
-                                   float(relaxation), n_steps, float(coarse_tol))
+                 coloring=COLOR_REDBLACK, relaxation=1.0, n_steps=1, coarse_tol=1e-12,
+                 vertices=None):
+        vp = None
+        if vertices is not None:
+            self._v = np.ascontiguousarray(vertices, dtype=np.float64)
+            vp = dp(self._v)
+        self.h = lib.or_gmg_create_v(dim, p, n_coarse, n_levels, float(ip), variant, coloring,
+                                     float(relaxation), n_steps, float(coarse_tol), vp)
         self.N = (n_coarse << (n_levels - 1)) ** dim * (p + 1) ** dim
 
     def __del__(self):

@@ -719,6 +719,7 @@ int or_setup(or_ctx *c)
   /* local solvers: FD::MatrixIntegrator::assemble_subspace_inverses,
    * apps/laplace_integrator.h:226-250 */
   int status = 0;
+#pragma omp parallel for num_threads(c->n_threads) schedule(static) reduction(| : status)
   for(int j = 0; j < P; ++j)
   {
     const int *v = c->vtx + j * dim;
@@ -1128,6 +1129,32 @@ static void operator_distorted(const or_ctx *c, double *dst, const double *src)
         if(interior) cn[d] += 1;
         double phim[512], dphim[512 * 3], phip[512], dphip[512 * 3];
         double Xm[3], Jm[9], Jim[9], Xp[3], Jp[9], Jip[9];
+        /* the penalty uses h = 1/|(J^{-1} n)_d| of both cells at the first
+         * face quadrature point only (normal_vector(0) * inverse_jacobian(0),
+         * apps/laplace_integrator.h:651-656, 692-693) */
+        double hm0, hp0 = 1.0;
+        {
+          double xim[3], xip[3], nv[3], nn = 0.0, s = 0.0;
+          for(int e = 0; e < dim; ++e)
+          {
+            xim[e] = e == d ? (double)side : qx[0];
+            xip[e] = e == d ? 0.0 : qx[0];
+          }
+          cell_map(c, cc, xim, Xm, Jm);
+          inv_det(dim, Jm, Jim);
+          for(int a = 0; a < dim; ++a) { nv[a] = Jim[d * dim + a] * (side == 0 ? -1.0 : 1.0); nn += nv[a] * nv[a]; }
+          nn = sqrt(nn);
+          for(int a = 0; a < dim; ++a) { nv[a] /= nn; s += nv[a] * Jim[d * dim + a]; }
+          hm0 = 1.0 / fabs(s);
+          if(interior)
+          {
+            cell_map(c, cn, xip, Xp, Jp);
+            inv_det(dim, Jp, Jip);
+            s = 0.0;
+            for(int a = 0; a < dim; ++a) s += nv[a] * Jip[d * dim + a];
+            hp0 = 1.0 / fabs(s);
+          }
+        }
         for(int q = 0; q < nqf; ++q)
         {
           double xim[3], xip[3], w = 1.0;
@@ -1150,13 +1177,7 @@ static void operator_distorted(const or_ctx *c, double *dst, const double *src)
           nn = sqrt(nn);
           for(int a = 0; a < dim; ++a) nv[a] /= nn;
           const double dS = fabs(detm) * nn * w;
-          /* h_m = 1/|n^T J^{-1}|_d  (component normal to the face) */
-          double hm;
-          {
-            double s = 0.0;
-            for(int a = 0; a < dim; ++a) s += nv[a] * Jim[d * dim + a];
-            hm = 1.0 / fabs(s);
-          }
+          const double hm = hm0;
           double um = 0.0, dnum = 0.0;
           double gm[512];
           for(int i = 0; i < ndc; ++i)
@@ -1182,12 +1203,7 @@ static void operator_distorted(const or_ctx *c, double *dst, const double *src)
           cell_map(c, cn, xip, Xp, Jp);
           inv_det(dim, Jp, Jip);
           cell_shape(c, nodes, xip, phip, dphip);
-          double hp;
-          {
-            double s = 0.0;
-            for(int a = 0; a < dim; ++a) s += nv[a] * Jip[d * dim + a];
-            hp = 1.0 / fabs(s);
-          }
+          const double hp = hp0;
           double up = 0.0, dnup = 0.0;
           double gp[512];
           for(int i = 0; i < ndc; ++i)
@@ -1312,6 +1328,30 @@ or_gmg *or_gmg_create(int dim, int p, int n_coarse, int n_levels, double ip,
                       int variant, int coloring, double relaxation,
                       int n_steps, double coarse_tol)
 {
+  return or_gmg_create_v(dim, p, n_coarse, n_levels, ip, variant, coloring, relaxation, n_steps,
+                         coarse_tol, NULL);
+}
+
+/* level vertices of a uniformly refined hierarchy: every second vertex of the
+ * next finer level */
+static double *coarsen_vertices(int dim, int nf, const double *vf)
+{
+  const int nc = nf / 2;
+  const long nvc = ipow(nc + 1, dim);
+  double *vc = (double *)malloc(sizeof(double) * nvc * dim);
+  for(long i = 0; i < nvc; ++i)
+  {
+    long t = i, fi = 0, st = 1;
+    for(int d = 0; d < dim; ++d) { fi += 2 * (t % (nc + 1)) * st; t /= (nc + 1); st *= (nf + 1); }
+    for(int d = 0; d < dim; ++d) vc[i * dim + d] = vf[fi * dim + d];
+  }
+  return vc;
+}
+
+or_gmg *or_gmg_create_v(int dim, int p, int n_coarse, int n_levels, double ip,
+                        int variant, int coloring, double relaxation,
+                        int n_steps, double coarse_tol, const double *vertices)
+{
   or_gmg *g = (or_gmg *)calloc(1, sizeof(or_gmg));
   g->dim = dim;
   g->p = p;
@@ -1323,6 +1363,15 @@ or_gmg *or_gmg_create(int dim, int p, int n_coarse, int n_levels, double ip,
   g->x = (double **)calloc(n_levels, sizeof(double *));
   g->b = (double **)calloc(n_levels, sizeof(double *));
   g->t = (double **)calloc(n_levels, sizeof(double *));
+  double **verts = (double **)calloc(n_levels, sizeof(double *));
+  if(vertices)
+  {
+    const int nf = n_coarse << (n_levels - 1);
+    const long nv = ipow(nf + 1, dim);
+    verts[n_levels - 1] = (double *)malloc(sizeof(double) * nv * dim);
+    memcpy(verts[n_levels - 1], vertices, sizeof(double) * nv * dim);
+    for(int l = n_levels - 2; l >= 0; --l) verts[l] = coarsen_vertices(dim, n_coarse << (l + 1), verts[l + 1]);
+  }
   for(int l = 0; l < n_levels; ++l)
   {
     const int n = n_coarse << l;
@@ -1330,6 +1379,12 @@ or_gmg *or_gmg_create(int dim, int p, int n_coarse, int n_levels, double ip,
      * (apps/poisson_standard.cc:122-123, apps/multigrid.h:304-353) */
     g->pre[l] = or_create(dim, p, n, ip, variant, coloring, relaxation, 0, 0, 1.0, n_steps);
     g->post[l] = or_create(dim, p, n, ip, variant, coloring, relaxation, 1, 0, 1.0, n_steps);
+    if(verts[l])
+    {
+      or_set_vertices(g->pre[l], verts[l]);
+      or_set_vertices(g->post[l], verts[l]);
+      free(verts[l]);
+    }
     or_setup(g->pre[l]);
     or_setup(g->post[l]);
     const long N = g->pre[l]->N;
@@ -1338,6 +1393,7 @@ or_gmg *or_gmg_create(int dim, int p, int n_coarse, int n_levels, double ip,
     g->b[l] = (double *)calloc(N, sizeof(double));
     g->t[l] = (double *)calloc(N, sizeof(double));
   }
+  free(verts);
   return g;
 }
 

@@ -79,6 +79,9 @@ typedef struct or_gmg or_gmg;
 or_gmg *or_gmg_create(int dim, int p, int n_coarse, int n_levels, double ip,
                       int variant, int coloring, double relaxation,
                       int n_steps, double coarse_tol);
+or_gmg *or_gmg_create_v(int dim, int p, int n_coarse, int n_levels, double ip,
+                        int variant, int coloring, double relaxation,
+                        int n_steps, double coarse_tol, const double *vertices);
 void or_gmg_destroy(or_gmg *g);
 or_ctx *or_gmg_level(or_gmg *g, int level);
 void or_gmg_vcycle(or_gmg *g, double *dst, const double *src);

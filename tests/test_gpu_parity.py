@@ -208,3 +208,56 @@ def test_gmg_mixed_precision_reaches_tolerance():
     assert hist[-1] <= 1e-8 * hist[0]
     lv = o.Smoother(3, 3, 8)
     assert np.linalg.norm(b - lv.operator(host(x))) <= 1.01e-8 * np.linalg.norm(b)
+
+
+# ---------------------------------------------------------------- distorted meshes
+
+@pytest.mark.parametrize("dim,p,n", [(2, 2, 5), (2, 3, 4), (3, 1, 3), (3, 2, 3), (3, 3, 4),
+                                     (3, 4, 3)])
+def test_distorted_operator_parity(dim, p, n):
+    t = _tpss()
+    V = o.distorted_vertices(dim, n, 0.2, seed=3)
+    lv = t.Level(dim, p, n, ip_factor=4.0, vertices=V)
+    ref = o.Smoother(dim, p, n, ip=4.0, vertices=V)
+    x = o.splitmix_uniform(1, ref.N)
+    y = dev(np.zeros_like(x))
+    lv.vmult(y, dev(x))
+    assert rel_err(host(y), ref.operator(x)) < 1e-12
+    b = o.splitmix_uniform(2, ref.N)
+    lv.residual(y, dev(b), dev(x))
+    assert rel_err(host(y), b - ref.operator(x)) < 1e-12
+
+
+@pytest.mark.parametrize("dim,p,n,variant", [(2, 3, 6, 0), (3, 3, 4, 0), (3, 2, 5, 1), (3, 4, 3, 0)])
+def test_distorted_smoother_parity(dim, p, n, variant):
+    V = o.distorted_vertices(dim, n, 0.15, seed=3)
+    lv, sm, ref = make(dim, p, n, variant, ip=4.0, vertices=V)
+    np.testing.assert_allclose(sm.patch_h(), ref.patch_h(), rtol=1e-13, atol=0)
+    src = o.splitmix_uniform(4, ref.N)
+    d = dev(np.zeros(ref.N))
+    sm.vmult(d, dev(src))
+    assert rel_err(host(d), ref.smoother_vmult(src)) < 1e-12
+
+
+def test_distorted_gmg_matches_oracle():
+    import tpss_amd as t
+    dim, p, levels = 3, 2, 3
+    V = o.distorted_vertices(dim, 2 << (levels - 1), 0.2, seed=3)
+    # AVP: one V-cycle (the surrogate AVP with the fixed reference damping does
+    # not make a convergent CG preconditioner on this mesh, in the oracle too)
+    g = t.GMGSolver(dim, p, 2, levels, ip_factor=4.0, vertices=V, variant=t.ADDITIVE,
+                    relaxation=0.15)
+    ref = o.GMG(dim, p, 2, levels, ip=4.0, variant=t.ADDITIVE, coloring=t.COLOR_NONE,
+                relaxation=0.15, vertices=V)
+    b = o.splitmix_uniform(2, g.n_dofs)
+    z = dev(np.zeros(g.n_dofs))
+    g.vcycle(z, dev(b))
+    assert rel_err(host(z), ref.vcycle(b)) < 1e-11
+    # MVP: full GMG-CG
+    g = t.GMGSolver(dim, p, 2, levels, ip_factor=4.0, vertices=V)
+    ref = o.GMG(dim, p, 2, levels, ip=4.0, vertices=V)
+    x = dev(np.zeros(g.n_dofs))
+    info, hist = g.solve(x, dev(b), rel_tol=1e-8)
+    xr, itr, histr = ref.cg(b, rel_tol=1e-8)
+    assert info.iterations == itr
+    assert rel_err(host(x), xr) < 1e-9

@@ -378,3 +378,24 @@ def test_ksvd_exact_rank2():
     B1, B0, s = o.ksvd(A, 3, 4, 2, 5, 2)
     approx = np.kron(B1[0], B0[0]) + np.kron(B1[1], B0[1])
     np.testing.assert_allclose(approx, A, atol=1e-12)
+
+
+# ---------------------------------------------------------------- surrogate patch lengths
+
+def test_surrogate_h_affine_and_cartesian():
+    """mapping_info.templates.h:42-240: on Cartesian meshes h is the cell size;
+    on an affinely stretched mesh the stretched cell size; on a sheared mesh
+    the distance between opposite faces along reference lines (edge length)."""
+    dim, p, n = 3, 2, 4
+    sm = o.Smoother(dim, p, n, vertices=o.uniform_vertices(dim, n))
+    np.testing.assert_allclose(sm.patch_h(), 1.0 / n, rtol=1e-14)
+    V = o.uniform_vertices(dim, n)
+    Vs = V * np.array([1.0, 2.0, 0.5])
+    sm = o.Smoother(dim, p, n, vertices=Vs)
+    np.testing.assert_allclose(sm.patch_h()[:, 1, :], 2.0 / n, rtol=1e-14)
+    np.testing.assert_allclose(sm.patch_h()[:, 2, :], 0.5 / n, rtol=1e-14)
+    Vsh = V.copy()
+    Vsh[:, 0] += 0.3 * V[:, 1]  # shear: cell edges along y have length h*sqrt(1+0.09)
+    sm = o.Smoother(dim, p, n, vertices=Vsh)
+    np.testing.assert_allclose(sm.patch_h()[:, 1, :], np.sqrt(1.09) / n, rtol=1e-13)
+    np.testing.assert_allclose(sm.patch_h()[:, 0, :], 1.0 / n, rtol=1e-13)

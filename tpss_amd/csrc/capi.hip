@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "setup_kernels.hpp"
 #include "ops.hpp"
+#include "kernels_general.hpp"
 
 namespace tpss
 {
@@ -50,23 +51,73 @@ void operator_apply(const tpss_level *lv, VT *y, const VT *x, const VT *b, const
                     int64_t n_list, hipStream_t s)
 {
   if(lv->distorted)
-    throw Error(TPSS_E_UNSUPPORTED, "SIPG operator on distorted meshes is not available on the device");
+  {
+    dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
+      constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
+      constexpr int NDC = cpow(NP, DIM);
+      GeneralTab<NP> tab;
+      const Basis1D bs = make_basis(NP - 1);
+      for(int q = 0; q < NP; ++q)
+      {
+        tab.qx[q] = bs.qx[q];
+        tab.qw[q] = bs.qw[q];
+        for(int i = 0; i < NP; ++i)
+        {
+          tab.V[q * NP + i] = bs.val[i * NP + q];
+          tab.Dv[q * NP + i] = bs.grad[i * NP + q];
+        }
+      }
+      for(int f = 0; f < 2; ++f)
+        for(int i = 0; i < NP; ++i)
+        {
+          tab.f[f][i] = bs.fval[i * 2 + f];
+          tab.fd[f][i] = bs.fgrad[i * 2 + f];
+        }
+      const int64_t count = cells ? n_list : lv->ncell;
+      if(count == 0) return;
+      const unsigned threads = NDC <= 64 ? 64u : 128u;
+      if(b)
+        hipLaunchKernelGGL((k_sipg_general<DIM, NP, VT, true>), dim3(unsigned(count)),
+                           dim3(threads), 0, s, x, y, b, lv->d_vertices.ptr, tab, lv->n, lv->ip,
+                           NP - 1, cells);
+      else
+        hipLaunchKernelGGL((k_sipg_general<DIM, NP, VT, false>), dim3(unsigned(count)),
+                           dim3(threads), 0, s, x, y, b, lv->d_vertices.ptr, tab, lv->n, lv->ip,
+                           NP - 1, cells);
+      TPSS_HIP(hipGetLastError());
+    });
+    return;
+  }
   dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
     constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
-    constexpr int CPB = cells_per_block<DIM, NP>();
-    constexpr int NDC = cpow(NP, DIM);
     using CT = VT;
     const auto tab = make_optab<CT, NP>(lv->op1d);
-    const int64_t count = cells ? n_list : lv->ncell;
-    if(count == 0) return;
-    const unsigned grid = blocks_for(count, CPB);
-    const unsigned threads = unsigned(((CPB * NDC + 63) / 64) * 64);
+    if(cells && n_list == 0) return;
+    if(cells)
+    {
+      // only the listed cells (the cells a multiplicative color reads)
+      constexpr int CPB = cells_per_block<DIM, NP>();
+      constexpr int NDC = cpow(NP, DIM);
+      const unsigned grid = blocks_for(n_list, CPB);
+      const unsigned threads = unsigned(((CPB * NDC + 63) / 64) * 64);
+      if(b)
+        hipLaunchKernelGGL((k_sipg_cartesian<DIM, NP, VT, CT, CPB, true>), dim3(grid),
+                           dim3(threads), 0, s, x, y, b, tab, lv->n, lv->ncell, cells, n_list);
+      else
+        hipLaunchKernelGGL((k_sipg_cartesian<DIM, NP, VT, CT, CPB, false>), dim3(grid),
+                           dim3(threads), 0, s, x, y, b, tab, lv->n, lv->ncell, cells, n_list);
+      TPSS_HIP(hipGetLastError());
+      return;
+    }
+    constexpr int TX = row_tile<DIM, NP>();
+    const int64_t rows = DIM == 3 ? int64_t(lv->n) * lv->n : lv->n;
+    const unsigned grid = unsigned(rows * ((lv->n + TX - 1) / TX));
     if(b)
-      hipLaunchKernelGGL((k_sipg_cartesian<DIM, NP, VT, CT, CPB, true>), dim3(grid), dim3(threads),
-                         0, s, x, y, b, tab, lv->n, lv->ncell, cells, n_list);
+      hipLaunchKernelGGL((k_sipg_rows<DIM, NP, VT, CT, true>), dim3(grid), dim3(256), 0, s, x, y, b,
+                         tab, lv->n);
     else
-      hipLaunchKernelGGL((k_sipg_cartesian<DIM, NP, VT, CT, CPB, false>), dim3(grid),
-                         dim3(threads), 0, s, x, y, b, tab, lv->n, lv->ncell, cells, n_list);
+      hipLaunchKernelGGL((k_sipg_rows<DIM, NP, VT, CT, false>), dim3(grid), dim3(256), 0, s, x, y,
+                         b, tab, lv->n);
     TPSS_HIP(hipGetLastError());
   });
 }

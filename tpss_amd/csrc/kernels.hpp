@@ -121,6 +121,150 @@ k_sipg_cartesian(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restr
   }
 }
 
+// Row-tiled variant: a block owns TX consecutive cells of one x-row and stages
+// that row (with its two x-halo cells) and the four neighbouring rows (+-y,
+// +-z) in LDS with contiguous loads.  With all neighbours local the operator is
+// evaluated in sum-factorised form
+//   y = M_z (M_y z_0 + M_x z_1) + M_y M_x z_2    (3D)
+//   y = M_y z_0 + M_x z_1                        (2D)
+// where z_d is the 1D line operator along d (own cell + face neighbours).
+template<int DIM, int NP>
+constexpr int row_tile()
+{
+  return cpow(NP, DIM) >= 256 ? 1 : 256 / cpow(NP, DIM);
+}
+
+template<int DIM, int NP, typename VT, typename CT, bool RESIDUAL>
+__global__ void __launch_bounds__(256)
+k_sipg_rows(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
+            const OpTab<CT, NP> tab, const int n)
+{
+  constexpr int NDC = cpow(NP, DIM);
+  constexpr int TX = row_tile<DIM, NP>();
+  constexpr int NROW = 2 * DIM - 1;  // own row + (2 DIM - 2) neighbour rows
+  constexpr int TXH = TX + 2;
+  __shared__ CT xs[NROW][TXH][NDC];
+  __shared__ CT zs[DIM][TX][NDC];
+  __shared__ CT as[2][TX][NDC];
+
+  const int chunks = (n + TX - 1) / TX;
+  int64_t rid = blockIdx.x;
+  const int chunk = int(rid % chunks);
+  rid /= chunks;
+  const int c1 = int(rid % n);
+  const int c2 = DIM == 3 ? int(rid / n) : 0;
+  const int c0s = chunk * TX;
+  const int64_t nn = int64_t(n) * n;
+  const int64_t row_base = int64_t(c1) * n + (DIM == 3 ? int64_t(c2) * nn : 0);
+
+  // ---- stage rows in LDS (contiguous cells along x)
+  for(int idx = threadIdx.x; idx < NROW * TXH * NDC; idx += blockDim.x)
+  {
+    const int r = idx / (TXH * NDC);
+    const int rem = idx % (TXH * NDC);
+    const int hc = rem / NDC, l = rem % NDC;
+    const int c0 = c0s - 1 + hc;
+    int64_t base = -1;
+    if(c0 >= 0 && c0 < n)
+    {
+      if(r == 0)
+        base = row_base + c0;
+      else if(r == 1 && c1 > 0)
+        base = row_base - n + c0;
+      else if(r == 2 && c1 < n - 1)
+        base = row_base + n + c0;
+      else if(r == 3 && c2 > 0)
+        base = row_base - nn + c0;
+      else if(r == 4 && c2 < n - 1)
+        base = row_base + nn + c0;
+    }
+    xs[r][hc][l] = base >= 0 ? CT(x[base * NDC + l]) : CT(0);
+  }
+  __syncthreads();
+
+  const int cs = threadIdx.x / NDC, li = threadIdx.x % NDC;
+  const bool in_block = cs < TX;
+  const int csafe = in_block ? cs : 0;
+  const int c0 = c0s + csafe;
+  const bool active = in_block && c0 < n;
+  int lid[3] = {0, 0, 0};
+  {
+    int t = li;
+    for(int d = 0; d < DIM; ++d)
+    {
+      lid[d] = t % NP;
+      t /= NP;
+    }
+  }
+  const int cid[3] = {c0, c1, c2};
+  constexpr int LS[3] = {1, NP, NP * NP};
+
+  // ---- 1D line operators z_d
+#pragma unroll
+  for(int d = 0; d < DIM; ++d)
+  {
+    const bool has_l = cid[d] > 0, has_r = cid[d] < n - 1;
+    const CT *D = tab.D[(has_l ? 1 : 0) | (has_r ? 2 : 0)];
+    const int i = lid[d];
+    const int base = li - i * LS[d];
+    const CT *own = xs[0][csafe + 1];
+    const CT *lft = d == 0 ? xs[0][csafe] : xs[2 * d - 1][csafe + 1];
+    const CT *rgt = d == 0 ? xs[0][csafe + 2] : xs[2 * d][csafe + 1];
+    CT z = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+    {
+      const int o = base + j * LS[d];
+      z += D[i * NP + j] * own[o];
+      // outside the domain the staged rows hold zeros
+      z += tab.Fpm[i * NP + j] * lft[o] + tab.Fmp[i * NP + j] * rgt[o];
+    }
+    if(in_block) zs[d][cs][li] = z;
+  }
+  __syncthreads();
+
+  CT out = 0;
+  if(DIM == 2)
+  {
+    const CT *z0 = zs[0][csafe], *z1 = zs[DIM > 1 ? 1 : 0][csafe];
+    const int b1 = li - lid[1] * NP, b0 = li - lid[0];
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+      out += tab.M[lid[1] * NP + j] * z0[b1 + j * NP] + tab.M[lid[0] * NP + j] * z1[b0 + j];
+  }
+  else
+  {
+    const CT *z0 = zs[0][csafe], *z1 = zs[1][csafe], *z2 = zs[DIM - 1][csafe];
+    const int b0 = li - lid[0], b1 = li - lid[1] * NP;
+    CT a = 0, bb = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+    {
+      a += tab.M[lid[1] * NP + j] * z0[b1 + j * NP] + tab.M[lid[0] * NP + j] * z1[b0 + j];
+      bb += tab.M[lid[0] * NP + j] * z2[b0 + j];
+    }
+    if(in_block)
+    {
+      as[0][cs][li] = a;
+      as[1][cs][li] = bb;
+    }
+    __syncthreads();
+    const CT *a0 = as[0][csafe], *a1 = as[1][csafe];
+    const int b2 = li - lid[2] * NP * NP;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+      out += tab.M[lid[2] * NP + j] * a0[b2 + j * NP * NP] + tab.M[lid[1] * NP + j] * a1[b1 + j * NP];
+  }
+  if(active)
+  {
+    const int64_t g = (row_base + c0) * NDC + li;
+    if(RESIDUAL)
+      y[g] = VT(CT(b[g]) - out);
+    else
+      y[g] = VT(out);
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Fused vertex-patch local solve (A2/A3/A4/A5/A12):
 //   gather r_j = R_j src  ->  u_j = S diag(omega_j / sum lambda) S^T r_j  ->

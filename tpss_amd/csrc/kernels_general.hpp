@@ -1,0 +1,409 @@
+// kernels_general.hpp -- SIPG Poisson operator on meshes with moved vertices
+// ((bi/tri)linear cells), matrix-free with QGauss(p+1) on cells and faces:
+// Laplace::DG::MF::Operator::{apply_cell, apply_face, apply_boundary}
+// (apps/laplace_integrator.h:613-743).  The geometry (Jacobians at quadrature
+// points) is recomputed from the cell vertices on the fly, so the only HBM
+// traffic is the vectors and the (n+1)^dim vertex array.
+//
+// One block per cell, one thread per cell DoF / quadrature point (nq = np).
+// Faces are evaluated element-centrically: every cell integrates its own test
+// functions against the flux of each of its 2*dim faces (interior faces are
+// thus evaluated from both sides, which is what makes the kernel write-race free).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace tpss
+{
+template<int NP>
+struct GeneralTab
+{
+  double V[NP * NP];   // V[q*NP + i]  = phi_i(x_q), Gauss points x_q
+  double Dv[NP * NP];  // Dv[q*NP + i] = phi_i'(x_q)
+  double qx[NP], qw[NP];
+  double f[2][NP];     // phi_i(0), phi_i(1)
+  double fd[2][NP];    // phi_i'(0), phi_i'(1)
+};
+
+// J[a][b] = d x_a / d xi_b of the multilinear map with corner coordinates X
+template<int DIM>
+__device__ __forceinline__ void map_jacobian(const double (*X)[3], const double *xi,
+                                             double (*J)[3])
+{
+  for(int a = 0; a < DIM; ++a)
+    for(int b = 0; b < DIM; ++b) J[a][b] = 0.0;
+  for(int c = 0; c < (1 << DIM); ++c)
+  {
+    for(int b = 0; b < DIM; ++b)
+    {
+      double w = 1.0;
+      for(int e = 0; e < DIM; ++e)
+      {
+        const int bit = (c >> e) & 1;
+        if(e == b)
+          w *= bit ? 1.0 : -1.0;
+        else
+          w *= bit ? xi[e] : 1.0 - xi[e];
+      }
+      for(int a = 0; a < DIM; ++a) J[a][b] += w * X[c][a];
+    }
+  }
+}
+
+// Ji = J^{-1}, returns det J
+template<int DIM>
+__device__ __forceinline__ double invert(const double (*J)[3], double (*Ji)[3])
+{
+  if(DIM == 2)
+  {
+    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    Ji[0][0] = J[1][1] / det;
+    Ji[0][1] = -J[0][1] / det;
+    Ji[1][0] = -J[1][0] / det;
+    Ji[1][1] = J[0][0] / det;
+    return det;
+  }
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double id = 1.0 / det;
+  Ji[0][0] = c00 * id;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  Ji[1][0] = c01 * id;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  Ji[2][0] = c02 * id;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  return det;
+}
+
+template<int DIM>
+__device__ __forceinline__ void load_corners(const double *__restrict__ vert, const int n,
+                                             const int *cc, double (*X)[3], int lane)
+{
+  // lanes 0 .. 2^DIM*DIM-1 load one coordinate each
+  if(lane < (1 << DIM) * DIM)
+  {
+    const int c = lane / DIM, a = lane % DIM;
+    int64_t idx = 0, st = 1;
+    for(int e = 0; e < DIM; ++e)
+    {
+      idx += int64_t(cc[e] + ((c >> e) & 1)) * st;
+      st *= (n + 1);
+    }
+    X[c][a] = vert[idx * DIM + a];
+  }
+}
+
+// contraction of an LDS tensor along direction e: out(t) = sum_j M[o*NP + j] src(t|e->j),
+// o = t_e; with TRANS the matrix is read as M[j*NP + o]
+template<int DIM, int NP, bool TRANS>
+__device__ __forceinline__ double contract(const double *src, const double *M, const int *tid,
+                                           const int t, const int e)
+{
+  constexpr int ST[3] = {1, NP, NP * NP};
+  const int o = tid[e];
+  const int base = t - o * ST[e];
+  double s = 0.0;
+#pragma unroll
+  for(int j = 0; j < NP; ++j) s += (TRANS ? M[j * NP + o] : M[o * NP + j]) * src[base + j * ST[e]];
+  return s;
+}
+
+template<int DIM, int NP, typename VT, bool RESIDUAL>
+__global__ void __launch_bounds__(128)
+k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
+               const double *__restrict__ vert, const GeneralTab<NP> tab, const int n,
+               const double ip, const int p, const uint32_t *__restrict__ cells)
+{
+  constexpr int NDC = cpow(NP, DIM);
+  constexpr int NF = cpow(NP, DIM - 1);
+  __shared__ double U[NDC], UN[NDC], A[NDC], B[NDC];
+  __shared__ double F[DIM][NDC];
+  __shared__ double X[8][3], XN[8][3];
+  __shared__ double FV[NF], FH[DIM][NF];
+
+  const int64_t cell = cells ? int64_t(cells[blockIdx.x]) : int64_t(blockIdx.x);
+  const int t = threadIdx.x;
+  const bool act = t < NDC;
+  const int ts = act ? t : 0;
+  int cc[3] = {0, 0, 0};
+  {
+    int64_t r = cell;
+    for(int d = 0; d < DIM; ++d)
+    {
+      cc[d] = int(r % n);
+      r /= n;
+    }
+  }
+  int tid[3] = {0, 0, 0};
+  {
+    int r = ts;
+    for(int d = 0; d < DIM; ++d)
+    {
+      tid[d] = r % NP;
+      r /= NP;
+    }
+  }
+  if(act) U[t] = double(x[cell * NDC + t]);
+  load_corners<DIM>(vert, n, cc, X, t);
+  __syncthreads();
+
+  // ---------------- cell term: reference gradient at the quadrature point t
+  double g[3] = {0, 0, 0};
+#pragma unroll
+  for(int a = 0; a < DIM; ++a)
+  {
+    // contract along every direction e with Dv (e == a) or V (e != a)
+    const double *src = U;
+#pragma unroll
+    for(int e = 0; e < DIM; ++e)
+    {
+      const double v = contract<DIM, NP, false>(src, e == a ? tab.Dv : tab.V, tid, ts, e);
+      double *dst = (e % 2 == 0) ? A : B;
+      __syncthreads();
+      if(act) dst[t] = v;
+      __syncthreads();
+      src = dst;
+    }
+    g[a] = src[ts];
+    __syncthreads();
+  }
+  {
+    double xi[3], J[3][3], Ji[3][3];
+    double w = 1.0;
+    for(int e = 0; e < DIM; ++e)
+    {
+      xi[e] = tab.qx[tid[e]];
+      w *= tab.qw[tid[e]];
+    }
+    map_jacobian<DIM>(X, xi, J);
+    const double det = invert<DIM>(J, Ji);
+    const double jxw = fabs(det) * w;
+    // flux in reference coordinates: Ji Ji^T g |det J| w
+    double ph[3];
+    for(int c = 0; c < DIM; ++c)
+    {
+      double s = 0.0;
+      for(int bb = 0; bb < DIM; ++bb) s += Ji[bb][c] * g[bb];
+      ph[c] = s;  // physical gradient
+    }
+    for(int a = 0; a < DIM; ++a)
+    {
+      double s = 0.0;
+      for(int c = 0; c < DIM; ++c) s += Ji[a][c] * ph[c];
+      if(act) F[a][t] = s * jxw;
+    }
+  }
+  __syncthreads();
+  double acc = 0.0;
+#pragma unroll
+  for(int a = 0; a < DIM; ++a)
+  {
+    const double *src = F[a];
+#pragma unroll
+    for(int e = 0; e < DIM; ++e)
+    {
+      const double v = contract<DIM, NP, true>(src, e == a ? tab.Dv : tab.V, tid, ts, e);
+      double *dst = (e % 2 == 0) ? A : B;
+      __syncthreads();
+      if(act) dst[t] = v;
+      __syncthreads();
+      src = dst;
+    }
+    acc += src[ts];
+    __syncthreads();
+  }
+
+  // ---------------- face terms
+  const double gamma = p == 0 ? 1.0 : double(p) * (p + 1);
+  int64_t cstride = 1;
+  for(int d = 0; d < DIM; ++d)
+  {
+    int e1 = -1, e2 = -1;
+    for(int e = 0; e < DIM; ++e)
+      if(e != d)
+      {
+        if(e1 < 0)
+          e1 = e;
+        else
+          e2 = e;
+      }
+    for(int side = 0; side < 2; ++side)
+    {
+      const bool interior = side == 1 ? cc[d] < n - 1 : cc[d] > 0;
+      const int64_t nb = interior ? (side == 1 ? cell + cstride : cell - cstride) : cell;
+      const int nside = 1 - side;
+      if(interior)
+      {
+        int cn[3] = {cc[0], cc[1], cc[2]};
+        cn[d] += side == 1 ? 1 : -1;
+        if(act) UN[t] = double(x[nb * NDC + t]);
+        load_corners<DIM>(vert, n, cn, XN, t);
+      }
+      __syncthreads();
+      if(t < NF)
+      {
+        const int l1 = t % NP, l2 = DIM == 3 ? t / NP : 0;
+        double xio[3], xin[3], wq = tab.qw[l1] * (DIM == 3 ? tab.qw[l2] : 1.0);
+        xio[d] = double(side);
+        xin[d] = double(nside);
+        xio[e1] = xin[e1] = tab.qx[l1];
+        if(DIM == 3) xio[e2] = xin[e2] = tab.qx[l2];
+        // trace values and reference gradients (own: g_o, neighbour: g_n)
+        double uo = 0, un = 0, go[3] = {0, 0, 0}, gn[3] = {0, 0, 0};
+        constexpr int ST[3] = {1, NP, NP * NP};
+        for(int k = 0; k < NDC; ++k)
+        {
+          int kk[3] = {0, 0, 0};
+          int r = k;
+          for(int e = 0; e < DIM; ++e)
+          {
+            kk[e] = r % NP;
+            r /= NP;
+          }
+          (void)ST;
+          const double t1v = tab.V[l1 * NP + kk[e1]], t1d = tab.Dv[l1 * NP + kk[e1]];
+          const double t2v = DIM == 3 ? tab.V[l2 * NP + kk[e2]] : 1.0;
+          const double t2d = DIM == 3 ? tab.Dv[l2 * NP + kk[e2]] : 0.0;
+          {
+            const double fv = tab.f[side][kk[d]], fdv = tab.fd[side][kk[d]];
+            const double val = U[k];
+            uo += fv * t1v * t2v * val;
+            go[d] += fdv * t1v * t2v * val;
+            go[e1] += fv * t1d * t2v * val;
+            if(DIM == 3) go[e2] += fv * t1v * t2d * val;
+          }
+          if(interior)
+          {
+            const double fv = tab.f[nside][kk[d]], fdv = tab.fd[nside][kk[d]];
+            const double val = UN[k];
+            un += fv * t1v * t2v * val;
+            gn[d] += fdv * t1v * t2v * val;
+            gn[e1] += fv * t1d * t2v * val;
+            if(DIM == 3) gn[e2] += fv * t1v * t2d * val;
+          }
+        }
+        double J[3][3], Jio[3][3], Jin[3][3];
+        map_jacobian<DIM>(X, xio, J);
+        const double deto = invert<DIM>(J, Jio);
+        // outward unit normal of this cell and surface element (Nanson)
+        double nv[3], nn = 0.0;
+        for(int a = 0; a < DIM; ++a)
+        {
+          nv[a] = Jio[d][a] * (side == 0 ? -1.0 : 1.0);
+          nn += nv[a] * nv[a];
+        }
+        nn = sqrt(nn);
+        for(int a = 0; a < DIM; ++a) nv[a] /= nn;
+        const double dS = fabs(deto) * nn * wq;
+        double co[3], cn[3] = {0, 0, 0};
+        for(int bb = 0; bb < DIM; ++bb)
+        {
+          double s = 0.0;
+          for(int a = 0; a < DIM; ++a) s += Jio[bb][a] * nv[a];
+          co[bb] = s;
+        }
+        double dno = 0.0;
+        for(int bb = 0; bb < DIM; ++bb) dno += go[bb] * co[bb];
+        double dnn = 0.0;
+        if(interior)
+        {
+          map_jacobian<DIM>(XN, xin, J);
+          invert<DIM>(J, Jin);
+          for(int bb = 0; bb < DIM; ++bb)
+          {
+            double s = 0.0;
+            for(int a = 0; a < DIM; ++a) s += Jin[bb][a] * nv[a];
+            cn[bb] = s;
+          }
+          for(int bb = 0; bb < DIM; ++bb) dnn += gn[bb] * cn[bb];
+        }
+        // penalty from the first face quadrature point (h = 1/|(J^{-1} n)_d|)
+        double ho, hn = 1.0;
+        {
+          double xi0[3], J0[3][3], Ji0[3][3], n0[3], nn0 = 0.0;
+          for(int e = 0; e < DIM; ++e) xi0[e] = e == d ? double(side) : tab.qx[0];
+          map_jacobian<DIM>(X, xi0, J0);
+          invert<DIM>(J0, Ji0);
+          for(int a = 0; a < DIM; ++a)
+          {
+            n0[a] = Ji0[d][a] * (side == 0 ? -1.0 : 1.0);
+            nn0 += n0[a] * n0[a];
+          }
+          nn0 = sqrt(nn0);
+          double s = 0.0;
+          for(int a = 0; a < DIM; ++a)
+          {
+            n0[a] /= nn0;
+            s += Ji0[d][a] * n0[a];
+          }
+          ho = 1.0 / fabs(s);
+          if(interior)
+          {
+            xi0[d] = double(nside);
+            map_jacobian<DIM>(XN, xi0, J0);
+            invert<DIM>(J0, Ji0);
+            s = 0.0;
+            for(int a = 0; a < DIM; ++a) s += Ji0[d][a] * n0[a];
+            hn = 1.0 / fabs(s);
+          }
+        }
+        double fval, fnd;
+        if(interior)
+        {
+          const double sigma = ip * 0.5 * 2.0 * gamma * (0.5 / ho + 0.5 / hn);
+          const double jump = uo - un, avg = 0.5 * (dno + dnn);
+          fval = dS * (jump * sigma - avg);
+          fnd = dS * (-0.5 * jump);
+        }
+        else
+        {
+          const double sigma = ip * 2.0 * gamma / ho;
+          fval = dS * (uo * sigma - dno);
+          fnd = dS * (-uo);
+        }
+        FV[t] = fval;
+        for(int bb = 0; bb < DIM; ++bb) FH[bb][t] = fnd * co[bb];
+      }
+      __syncthreads();
+      if(act)
+      {
+        // test functions of this cell: phi_i = f(i_d) * tangential
+        const int id = tid[d], i1 = tid[e1], i2 = DIM == 3 ? tid[e2] : 0;
+        const double fv = tab.f[side][id], fdv = tab.fd[side][id];
+        double s = 0.0;
+        for(int l = 0; l < NF; ++l)
+        {
+          const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
+          const double t1v = tab.V[l1 * NP + i1], t1d = tab.Dv[l1 * NP + i1];
+          const double t2v = DIM == 3 ? tab.V[l2 * NP + i2] : 1.0;
+          const double t2d = DIM == 3 ? tab.Dv[l2 * NP + i2] : 0.0;
+          s += fv * t1v * t2v * FV[l];
+          s += fdv * t1v * t2v * FH[d][l];
+          s += fv * t1d * t2v * FH[e1][l];
+          if(DIM == 3) s += fv * t1v * t2d * FH[e2][l];
+        }
+        acc += s;
+      }
+      __syncthreads();
+    }
+    cstride *= n;
+  }
+  if(act)
+  {
+    if(RESIDUAL)
+      y[cell * NDC + t] = VT(double(b[cell * NDC + t]) - acc);
+    else
+      y[cell * NDC + t] = VT(acc);
+  }
+}
+
+}  // namespace tpss

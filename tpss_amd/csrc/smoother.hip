@@ -299,33 +299,50 @@ void avp_sum(tpss_smoother *sm, const CT *ubuf, VT *dst, bool accumulate, double
   });
 }
 
+// Device-time sections: a pair of HIP events around the section's launches on
+// the caller's stream; no synchronisation here, the pending pairs are resolved
+// by tpss_smoother_get_timings.
 struct SectionTimer
 {
   tpss_smoother *sm;
   int idx;
   hipStream_t s;
-  hipEvent_t a = nullptr, b = nullptr;
+  hipEvent_t a = nullptr;
   SectionTimer(tpss_smoother *sm_, int idx_, hipStream_t s_) : sm(sm_), idx(idx_), s(s_)
   {
     if(!sm->timing) return;
     TPSS_HIP(hipEventCreate(&a));
-    TPSS_HIP(hipEventCreate(&b));
     TPSS_HIP(hipEventRecord(a, s));
   }
   ~SectionTimer()
   {
-    if(!sm->timing) return;
-    float ms = 0.f;
-    if(hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
-       hipEventElapsedTime(&ms, a, b) == hipSuccess)
+    if(!sm->timing || !a) return;
+    hipEvent_t b = nullptr;
+    if(hipEventCreate(&b) != hipSuccess || hipEventRecord(b, s) != hipSuccess)
     {
-      sm->timers[idx].time += 1e-3 * ms;
-      sm->timers[idx].n_calls += 1;
+      (void)hipEventDestroy(a);
+      if(b) (void)hipEventDestroy(b);
+      return;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    sm->pending.push_back({idx, a, b});
   }
 };
+
+void resolve_timings(tpss_smoother *sm)
+{
+  for(auto &p : sm->pending)
+  {
+    float ms = 0.f;
+    if(hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+    {
+      sm->timers[p.idx].time += 1e-3 * ms;
+      sm->timers[p.idx].n_calls += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  sm->pending.clear();
+}
 
 // dst = P src  (or, if accumulate: dst += alpha * P src for the additive variant)
 template<typename VT, typename CT>
@@ -504,6 +521,7 @@ int tpss_smoother_clone_shallow(const tpss_smoother *other, const tpss_smoother_
 
 int tpss_smoother_destroy(tpss_smoother *sm)
 {
+  if(sm) resolve_timings(sm);
   delete sm;
   return TPSS_OK;
 }
@@ -612,6 +630,7 @@ int tpss_smoother_get_timings(const tpss_smoother *sm, tpss_time_info *out, int 
                                    "[SchwarzPrecond] Step:"};
     const int cap = *n;
     *n = 5;
+    resolve_timings(const_cast<tpss_smoother *>(sm));
     if(!out) return;
     for(int i = 0; i < std::min(cap, 5); ++i)
     {
@@ -626,6 +645,10 @@ int tpss_smoother_set_timing(tpss_smoother *sm, int enabled)
 {
   return guarded([&] {
     TPSS_REQUIRE(sm, "null argument");
+    resolve_timings(sm);
+    if(enabled && !sm->timing)
+      for(int i = 0; i < 5; ++i)
+        if(i != 2) sm->timers[i] = Timer{};
     sm->timing = enabled != 0;
   });
 }

@@ -48,6 +48,12 @@ struct tpss_smoother
   bool timing = false;
   bool check_finite = false;
   tpss::Timer timers[5];
+  struct PendingSection
+  {
+    int idx;
+    hipEvent_t a, b;
+  };
+  std::vector<PendingSection> pending;
   // scratch (bytes)
   tpss::DeviceArray<char> res, tmp2, rich_r, ubuf;
   tpss::DeviceArray<int> flag;
