@@ -135,6 +135,11 @@ int tpss_smoother_set_timing(tpss_smoother *sm, int enabled);
 /* always-on AssertIsFinite (schwarz_preconditioner.templates.h:238 is debug-only
  * in the reference); costs one device synchronisation per call */
 int tpss_smoother_set_check_finite(tpss_smoother *sm, int enabled);
+/* Multiplicative variant on Cartesian meshes with red-black colors: each color
+ * pass computes b_j - B_j x_out inside the patch kernel instead of a separate
+ * residual update (default on; 0 selects the residual-update path that the
+ * reference follows, schwarz_preconditioner.templates.h:719-731). */
+int tpss_smoother_set_fused_residual(tpss_smoother *sm, int enabled);
 
 /* ---- geometric multigrid + CG (caller of the smoother) ------------------- */
 /* ModelProblem::prepare_multigrid + solve (apps/poisson_problem.h:642-755),
@@ -170,6 +175,35 @@ int tpss_gmg_vcycle(tpss_gmg *g, void *dst, const void *src, void *stream);
 /* GMG-preconditioned CG in FP64 (x, b: FP64 device vectors of the fine level) */
 int tpss_gmg_solve(tpss_gmg *g, void *x, const void *b, double rel_tol, int max_iter,
                    void *stream, tpss_solve_info *info, double *residual_history);
+
+/* ---- learned per-patch damping ------------------------------------------- */
+/* No counterpart in the reference: the reference applies one scalar
+ * relaxation to every patch (schwarz_preconditioner.templates.h:631-633).
+ * An MLP maps the singular values of the 2^dim cell Jacobians of a patch to
+ * omega_j in [omega_min, omega_max]; BF16 weights, FP32 accumulation. */
+typedef struct tpss_mlp tpss_mlp;
+int tpss_mlp_create(int n_in, int n_hidden, double omega_min, double omega_max, uint64_t seed,
+                    tpss_mlp **out);
+int tpss_mlp_destroy(tpss_mlp *m);
+int tpss_mlp_n_params(const tpss_mlp *m, size_t *n);
+int tpss_mlp_set_params(tpss_mlp *m, const float *host, size_t n);
+int tpss_mlp_get_params(const tpss_mlp *m, float *host, size_t n);
+/* out[b] = omega(X[b, :]) for device X (batch x n_in) */
+int tpss_mlp_forward(tpss_mlp *m, const float *X, float *out, int64_t batch, void *stream);
+/* mean-squared loss 1/(2B) sum (omega - target)^2 and its gradient (device grad_out) */
+int tpss_mlp_loss_grad(tpss_mlp *m, const float *X, const float *target, int64_t batch,
+                       float *grad_out, double *loss, void *stream);
+/* Adam over (X, target) on the device, minibatches in a seeded order */
+int tpss_mlp_train(tpss_mlp *m, const float *X, const float *target, int64_t n, int epochs,
+                   int batch, double lr, uint64_t seed, double *final_loss);
+/* per-patch features (device float[n_patches * dim * 2^dim]) */
+int tpss_smoother_patch_features(const tpss_smoother *sm, float *dev_out, void *stream);
+/* per-patch optimal local damping omega_j = sum_e d_j^T r / sum_e d_j^T A d_j
+ * (d_j = R_j^T A_j^{-1} R_j r, r = A e, e random), clamped (device float[n_patches]) */
+int tpss_smoother_optimal_damping(tpss_smoother *sm, int n_samples, uint64_t seed,
+                                  double omega_min, double omega_max, float *dev_out);
+/* omega_j = MLP(features_j) for every patch (NULL restores the scalar relaxation) */
+int tpss_smoother_set_damping_mlp(tpss_smoother *sm, tpss_mlp *m);
 
 /* ---- grid transfer (MGTransferMatrixFree for FE_DGQ) --------------------- */
 int tpss_transfer_prolongate_add(tpss_level *coarse, tpss_level *fine, void *dst_fine,

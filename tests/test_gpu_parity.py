@@ -187,8 +187,17 @@ def test_gmg_cg_matches_oracle(dim, p, levels, variant):
     g = t.GMGSolver(dim, p, 2, levels, variant=variant, coloring=col, relaxation=relax)
     ref = o.GMG(dim, p, 2, levels, variant=variant, coloring=col, relaxation=relax)
     b = o.splitmix_uniform(2, g.n_dofs)
-    # one V-cycle
+    # one V-cycle (multiplicative: residual-update path first, then fused passes)
     z = dev(np.zeros(g.n_dofs))
+    if variant == 1:
+        for l in range(1, levels):
+            for post in (False, True):
+                g.smoother(l, post).set_fused_residual(False)
+        g.vcycle(z, dev(b))
+        assert rel_err(host(z), ref.vcycle(b)) < 1e-11, "residual-update path"
+        for l in range(1, levels):
+            for post in (False, True):
+                g.smoother(l, post).set_fused_residual(True)
     g.vcycle(z, dev(b))
     assert rel_err(host(z), ref.vcycle(b)) < 1e-11
     x = dev(np.zeros(g.n_dofs))
@@ -261,3 +270,24 @@ def test_distorted_gmg_matches_oracle():
     xr, itr, histr = ref.cg(b, rel_tol=1e-8)
     assert info.iterations == itr
     assert rel_err(host(x), xr) < 1e-9
+
+
+@pytest.mark.parametrize("dim,p,n,relax,rev", [(2, 3, 9, 1.0, False), (3, 3, 7, 1.0, False),
+                                               (3, 2, 6, 0.7, True), (2, 2, 8, 0.6, False),
+                                               (2, 3, 4, 1.0, False), (2, 3, 8, 1.0, True),
+                                               (2, 3, 16, 1.0, False), (3, 2, 4, 1.0, False)])
+def test_mvp_fused_and_residual_paths(dim, p, n, relax, rev):
+    """Residual-free color passes and the residual-update path both match the
+    oracle (which recomputes the full residual before every color)."""
+    import tpss_amd as t
+    lv, sm, ref = make(dim, p, n, t.MULTIPLICATIVE, relaxation=relax, reverse=rev)
+    src = o.splitmix_uniform(12, ref.N)
+    expect = ref.precond_vmult(src)
+    for fused in (True, False):
+        sm.set_fused_residual(fused)
+        d = dev(o.splitmix_uniform(13, ref.N))  # vmult overwrites whatever dst holds
+        sm.precond_vmult(d, dev(src))
+        assert rel_err(host(d), expect) < 1e-12, fused
+        # twice in a row (scratch buffers keep the previous call's contents)
+        sm.precond_vmult(d, dev(src))
+        assert rel_err(host(d), expect) < 1e-12, fused

@@ -74,11 +74,15 @@ EXPORTS = [
     "tpss_smoother_set_patch_damping", "tpss_smoother_info", "tpss_smoother_color_ptr",
     "tpss_smoother_patch_vertices", "tpss_smoother_patch_dof_indices", "tpss_smoother_patch_h",
     "tpss_smoother_get_timings", "tpss_smoother_set_timing", "tpss_smoother_set_check_finite",
+    "tpss_smoother_set_fused_residual",
     "tpss_gmg_create", "tpss_gmg_destroy", "tpss_gmg_level", "tpss_gmg_smoother",
     "tpss_gmg_vcycle", "tpss_gmg_solve", "tpss_transfer_prolongate_add",
     "tpss_transfer_restrict", "tpss_vector_fill_random", "tpss_vector_convert",
     "tpss_device_count", "tpss_device_malloc", "tpss_device_free", "tpss_memcpy_h2d",
     "tpss_memcpy_d2h", "tpss_device_synchronize",
+    "tpss_mlp_create", "tpss_mlp_destroy", "tpss_mlp_n_params", "tpss_mlp_set_params",
+    "tpss_mlp_get_params", "tpss_mlp_forward", "tpss_mlp_loss_grad", "tpss_mlp_train",
+    "tpss_smoother_patch_features", "tpss_smoother_optimal_damping", "tpss_smoother_set_damping_mlp",
 ]
 
 _VP = ctypes.c_void_p
@@ -114,6 +118,7 @@ def _load():
         "tpss_smoother_get_timings": (ctypes.c_int, [_VP, ctypes.POINTER(TimeInfo), ip]),
         "tpss_smoother_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
         "tpss_smoother_set_check_finite": (ctypes.c_int, [_VP, ctypes.c_int]),
+        "tpss_smoother_set_fused_residual": (ctypes.c_int, [_VP, ctypes.c_int]),
         "tpss_gmg_create": (ctypes.c_int, [ctypes.POINTER(GMGDesc), _PVP]),
         "tpss_gmg_destroy": (ctypes.c_int, [_VP]),
         "tpss_gmg_level": (ctypes.c_int, [_VP, ctypes.c_int, _PVP]),
@@ -132,6 +137,22 @@ def _load():
         "tpss_memcpy_h2d": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
         "tpss_memcpy_d2h": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
         "tpss_device_synchronize": (ctypes.c_int, []),
+        "tpss_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_uint64, _PVP]),
+        "tpss_mlp_destroy": (ctypes.c_int, [_VP]),
+        "tpss_mlp_n_params": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t)]),
+        "tpss_mlp_set_params": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]),
+        "tpss_mlp_get_params": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]),
+        "tpss_mlp_forward": (ctypes.c_int, [_VP, _VP, _VP, i64, _VP]),
+        "tpss_mlp_loss_grad": (ctypes.c_int, [_VP, _VP, _VP, i64, _VP,
+                                              ctypes.POINTER(ctypes.c_double), _VP]),
+        "tpss_mlp_train": (ctypes.c_int, [_VP, _VP, _VP, i64, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_double)]),
+        "tpss_smoother_patch_features": (ctypes.c_int, [_VP, _VP, _VP]),
+        "tpss_smoother_optimal_damping": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64,
+                                                         ctypes.c_double, ctypes.c_double, _VP]),
+        "tpss_smoother_set_damping_mlp": (ctypes.c_int, [_VP, _VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -336,8 +357,29 @@ class SchwarzPreconditioner:
         _check(lib.tpss_smoother_set_patch_damping(self.handle,
                                                    w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
 
+    def patch_features(self):
+        """Singular values of the cell Jacobians of every patch (scaled by n)."""
+        nf = self.level.dim * 2 ** self.level.dim
+        v = DeviceVector(self.n_patches * nf, F32)
+        _check(lib.tpss_smoother_patch_features(self.handle, _ptr(v), None))
+        return v.numpy().reshape(self.n_patches, nf)
+
+    def optimal_damping(self, n_samples=4, seed=5, omega_min=0.2, omega_max=1.2):
+        """Per-patch A-norm line-search damping computed with the smoother's own
+        local solvers and operator (FP64 levels only)."""
+        v = DeviceVector(self.n_patches, F32)
+        _check(lib.tpss_smoother_optimal_damping(self.handle, int(n_samples), int(seed),
+                                                 float(omega_min), float(omega_max), _ptr(v)))
+        return v.numpy()
+
+    def set_damping_mlp(self, mlp):
+        _check(lib.tpss_smoother_set_damping_mlp(self.handle, mlp.handle if mlp else None))
+
     def set_timing(self, enabled=True):
         _check(lib.tpss_smoother_set_timing(self.handle, int(enabled)))
+
+    def set_fused_residual(self, enabled=True):
+        _check(lib.tpss_smoother_set_fused_residual(self.handle, int(enabled)))
 
     def set_check_finite(self, enabled=True):
         _check(lib.tpss_smoother_set_check_finite(self.handle, int(enabled)))
@@ -456,6 +498,77 @@ def fill_random(x, seed, precision=None, stream=None):
     if precision is None:
         precision = getattr(x, "precision", F64)
     _check(lib.tpss_vector_fill_random(_ptr(x), x.numel(), precision, seed, _stream(stream)))
+
+
+class DampingMLP:
+    """Per-patch damping network (features -> H tanh -> H tanh -> sigmoid-scaled omega)."""
+
+    def __init__(self, n_in=24, n_hidden=64, omega_min=0.2, omega_max=1.2, seed=4):
+        h = ctypes.c_void_p()
+        _check(lib.tpss_mlp_create(n_in, n_hidden, float(omega_min), float(omega_max), int(seed),
+                                   ctypes.byref(h)))
+        self.handle = h
+        self.n_in, self.n_hidden, self.omega_min, self.omega_max = n_in, n_hidden, omega_min, omega_max
+        n = ctypes.c_size_t()
+        _check(lib.tpss_mlp_n_params(h, ctypes.byref(n)))
+        self.n_params = n.value
+
+    def get_params(self):
+        import numpy as np
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib.tpss_mlp_get_params(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       out.size))
+        return out
+
+    def set_params(self, params):
+        import numpy as np
+        w = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib.tpss_mlp_set_params(self.handle, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       w.size))
+
+    def forward(self, X):
+        """X: host (B, n_in) array -> host omega (B,)."""
+        import numpy as np
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        dX = DeviceVector.from_numpy(X.ravel(), F32)
+        out = DeviceVector(X.shape[0], F32)
+        _check(lib.tpss_mlp_forward(self.handle, _ptr(dX), _ptr(out), X.shape[0], None))
+        return out.numpy()
+
+    def loss_grad(self, X, target):
+        import numpy as np
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        dX = DeviceVector.from_numpy(X.ravel(), F32)
+        dt = DeviceVector.from_numpy(np.ascontiguousarray(target, dtype=np.float32), F32)
+        g = DeviceVector(self.n_params, F32)
+        loss = ctypes.c_double()
+        _check(lib.tpss_mlp_loss_grad(self.handle, _ptr(dX), _ptr(dt), X.shape[0], _ptr(g),
+                                      ctypes.byref(loss), None))
+        return loss.value, g.numpy()
+
+    def train(self, X, target, epochs=30, batch=4096, lr=2e-3, seed=1):
+        import numpy as np
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        dX = DeviceVector.from_numpy(X.ravel(), F32)
+        dt = DeviceVector.from_numpy(np.ascontiguousarray(target, dtype=np.float32), F32)
+        loss = ctypes.c_double()
+        _check(lib.tpss_mlp_train(self.handle, _ptr(dX), _ptr(dt), X.shape[0], int(epochs),
+                                  int(batch), float(lr), int(seed), ctypes.byref(loss)))
+        return loss.value
+
+    def save(self, path):
+        import numpy as np
+        np.save(path, self.get_params(), allow_pickle=False)
+
+    def load(self, path):
+        import numpy as np
+        self.set_params(np.load(path, allow_pickle=False))
+        return self
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.tpss_mlp_destroy(self.handle)
+            self.handle = None
 
 
 def version():

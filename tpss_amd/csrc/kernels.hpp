@@ -338,6 +338,142 @@ __global__ void k_patch_dof_indices(const uint32_t *__restrict__ vtx, const int 
   out[g] = uint32_t(patch_cell<DIM, NP>(v, n, i0, i1, i2) * NDC + patch_local<DIM, NP>(i0, i1, i2));
 }
 
+// Fast-diagonalization inverse of one patch held as x-lines: thread t owns the
+// x-line (y, z) = (t % K, t / K); on entry r is that line of the patch
+// right-hand side, on exit o is the same line of
+//   u = (S_2 x S_1 x S_0) diag(w / (lam_0 + lam_1 + lam_2)) (S_2^T x S_1^T x S_0^T) r.
+// The line direction rotates x -> y -> z -> y -> x through the patch's LDS
+// buffer B (K^DIM entries); contains __syncthreads (every thread of the block
+// must call it).
+template<int DIM, int K, typename CT>
+__device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, const CT *S1,
+                                         const CT *S2, const CT *l0, const CT *l1, const CT *l2,
+                                         const CT w, const int t, const bool owner)
+{
+  const int ty = DIM > 1 ? t % K : 0;
+  const int tz = DIM > 2 ? t / K : 0;
+  // forward x
+#pragma unroll
+  for(int i = 0; i < K; ++i)
+  {
+    CT s = 0;
+#pragma unroll
+    for(int a = 0; a < K; ++a) s += S0[a * K + i] * r[a];
+    o[i] = s;
+  }
+  if(DIM == 2)
+  {
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+      if(owner) B[i + K * ty] = o[i];
+    __syncthreads();
+    const int x0 = t;
+#pragma unroll
+    for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj];
+#pragma unroll
+    for(int jj = 0; jj < K; ++jj)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
+      o[jj] = s * w / (l0[x0] + l1[jj]);
+    }
+#pragma unroll
+    for(int b = 0; b < K; ++b)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj) s += S1[b * K + jj] * o[jj];
+      r[b] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for(int b = 0; b < K; ++b)
+      if(owner) B[x0 + K * b] = r[b];
+    __syncthreads();
+#pragma unroll
+    for(int i = 0; i < K; ++i) r[i] = B[i + K * ty];
+  }
+  else
+  {
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+      if(owner) B[i + K * ty + K * K * tz] = o[i];
+    __syncthreads();
+    {
+      const int x0 = t % K, z0 = t / K;
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj + K * K * z0];
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
+        o[jj] = s;
+      }
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj)
+        if(owner) B[x0 + K * jj + K * K * z0] = o[jj];
+    }
+    __syncthreads();
+    {
+      const int x0 = t % K, y0 = t / K;
+#pragma unroll
+      for(int l = 0; l < K; ++l) r[l] = B[x0 + K * y0 + K * K * l];
+      const CT lxy = l0[x0] + l1[y0];
+#pragma unroll
+      for(int l = 0; l < K; ++l)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int c = 0; c < K; ++c) s += S2[c * K + l] * r[c];
+        o[l] = s * w / (lxy + l2[l]);
+      }
+#pragma unroll
+      for(int c = 0; c < K; ++c)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int l = 0; l < K; ++l) s += S2[c * K + l] * o[l];
+        r[c] = s;
+      }
+#pragma unroll
+      for(int c = 0; c < K; ++c)
+        if(owner) B[x0 + K * y0 + K * K * c] = r[c];
+    }
+    __syncthreads();
+    {
+      const int x0 = t % K, z0 = t / K;
+#pragma unroll
+      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj + K * K * z0];
+#pragma unroll
+      for(int b = 0; b < K; ++b)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int jj = 0; jj < K; ++jj) s += S1[b * K + jj] * r[jj];
+        o[b] = s;
+      }
+#pragma unroll
+      for(int b = 0; b < K; ++b)
+        if(owner) B[x0 + K * b + K * K * z0] = o[b];
+    }
+    __syncthreads();
+#pragma unroll
+    for(int i = 0; i < K; ++i) r[i] = B[i + K * ty + K * K * tz];
+  }
+  // backward x
+#pragma unroll
+  for(int a = 0; a < K; ++a)
+  {
+    CT s = 0;
+#pragma unroll
+    for(int i = 0; i < K; ++i) s += S0[a * K + i] * r[i];
+    o[a] = s;
+  }
+}
+
 template<int DIM, int K, typename VT, typename CT, int MODE>
 __global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS))
 k_patch_fd(const VT *__restrict__ src, VT *__restrict__ dst, CT *__restrict__ ubuf,
@@ -394,129 +530,8 @@ k_patch_fd(const VT *__restrict__ src, VT *__restrict__ dst, CT *__restrict__ ub
   for(int i = 0; i < K; ++i)
     r[i] = valid ? CT(src[cell_of(i, ty, tz) * NDC + loc_of(i, ty, tz)]) : CT(0);
 
-  // ---- forward x
-#pragma unroll
-  for(int i = 0; i < K; ++i)
-  {
-    CT s = 0;
-#pragma unroll
-    for(int a = 0; a < K; ++a) s += S0[a * K + i] * r[a];
-    o[i] = s;
-  }
-  if(DIM == 2)
-  {
-#pragma unroll
-    for(int i = 0; i < K; ++i)
-      if(pl < PPB) B[i + K * ty] = o[i];
-    __syncthreads();
-    const int x0 = t;  // y-line at x = t
-#pragma unroll
-    for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj];
-#pragma unroll
-    for(int jj = 0; jj < K; ++jj)
-    {
-      CT s = 0;
-#pragma unroll
-      for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
-      o[jj] = s * w / (ls[plc][0][x0] + ls[plc][1][jj]);
-    }
-#pragma unroll
-    for(int b = 0; b < K; ++b)
-    {
-      CT s = 0;
-#pragma unroll
-      for(int jj = 0; jj < K; ++jj) s += S1[b * K + jj] * o[jj];
-      r[b] = s;
-    }
-#pragma unroll
-    for(int b = 0; b < K; ++b)
-      if(pl < PPB) B[x0 + K * b] = r[b];
-    __syncthreads();
-#pragma unroll
-    for(int i = 0; i < K; ++i) r[i] = B[i + K * ty];
-  }
-  else
-  {
-#pragma unroll
-    for(int i = 0; i < K; ++i)
-      if(pl < PPB) B[i + K * ty + K * K * tz] = o[i];
-    __syncthreads();
-    // y-lines: thread (x, z) = (t % K, t / K)
-    {
-      const int x0 = t % K, z0 = t / K;
-#pragma unroll
-      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj + K * K * z0];
-#pragma unroll
-      for(int jj = 0; jj < K; ++jj)
-      {
-        CT s = 0;
-#pragma unroll
-        for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
-        o[jj] = s;
-      }
-#pragma unroll
-      for(int jj = 0; jj < K; ++jj)
-        if(pl < PPB) B[x0 + K * jj + K * K * z0] = o[jj];
-    }
-    __syncthreads();
-    // z-lines: thread (x, y) = (t % K, t / K)
-    {
-      const int x0 = t % K, y0 = t / K;
-#pragma unroll
-      for(int l = 0; l < K; ++l) r[l] = B[x0 + K * y0 + K * K * l];
-      const CT lxy = ls[plc][0][x0] + ls[plc][1][y0];
-#pragma unroll
-      for(int l = 0; l < K; ++l)
-      {
-        CT s = 0;
-#pragma unroll
-        for(int c = 0; c < K; ++c) s += S2[c * K + l] * r[c];
-        o[l] = s * w / (lxy + ls[plc][2][l]);
-      }
-#pragma unroll
-      for(int c = 0; c < K; ++c)
-      {
-        CT s = 0;
-#pragma unroll
-        for(int l = 0; l < K; ++l) s += S2[c * K + l] * o[l];
-        r[c] = s;
-      }
-#pragma unroll
-      for(int c = 0; c < K; ++c)
-        if(pl < PPB) B[x0 + K * y0 + K * K * c] = r[c];
-    }
-    __syncthreads();
-    // backward y
-    {
-      const int x0 = t % K, z0 = t / K;
-#pragma unroll
-      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj + K * K * z0];
-#pragma unroll
-      for(int b = 0; b < K; ++b)
-      {
-        CT s = 0;
-#pragma unroll
-        for(int jj = 0; jj < K; ++jj) s += S1[b * K + jj] * r[jj];
-        o[b] = s;
-      }
-#pragma unroll
-      for(int b = 0; b < K; ++b)
-        if(pl < PPB) B[x0 + K * b + K * K * z0] = o[b];
-    }
-    __syncthreads();
-#pragma unroll
-    for(int i = 0; i < K; ++i) r[i] = B[i + K * ty + K * K * tz];
-  }
-
-  // ---- backward x and scatter
-#pragma unroll
-  for(int a = 0; a < K; ++a)
-  {
-    CT s = 0;
-#pragma unroll
-    for(int i = 0; i < K; ++i) s += S0[a * K + i] * r[i];
-    o[a] = s;
-  }
+  fd_lines<DIM, K, CT>(r, o, B, S0, S1, S2, ls[plc][0], ls[plc][1], ls[plc][DIM > 2 ? 2 : 1], w,
+                       t, pl < PPB);
   if(!valid) return;
 #pragma unroll
   for(int a = 0; a < K; ++a)

@@ -1,0 +1,245 @@
+// kernels_mvp.hpp -- residual-free multiplicative color pass on Cartesian meshes.
+//
+// For a patch j of the current color the reference computes
+//   r = b - A x,  u_j = A_j^{-1} R_j r,  x += omega R_j^T u_j
+// (schwarz_preconditioner.templates.h:585-662, 691-732).  On tensor-product
+// cells A_j is exactly the patch block R_j A R_j^T, so with B_j the coupling of
+// the patch to the cells just outside it,
+//   R_j r = b_j - A_j x_j - B_j x_out   and
+//   x_j <- (1 - omega) x_j + omega A_j^{-1} (b_j - B_j x_out).
+// B_j x_out is the face coupling of the SIPG operator through the patch's outer
+// faces: per patch cell and direction d (one outer face each) the neighbour
+// term T_d(F) x_nbr of the cell operator, evaluated here in sum-factorised form
+//   M_z (M_y z_0 + M_x z_1) + M_y M_x z_2,  z_d = F_d x_nbr(d) along d.
+// Valid when same-color patches are not face-coupled (red-black coloring), so
+// no patch of the pass reads cells another patch of the pass writes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace tpss
+{
+template<int DIM, int K, typename VT, typename CT>
+__global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS))
+k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, K / 2> tab,
+                  const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
+                  const CT *__restrict__ S_tab, const CT *__restrict__ lam_tab,
+                  const float *__restrict__ omega, const CT relaxation, const int patch_begin,
+                  const int patch_end, const int n, const int x_is_zero)
+{
+  using PS = PatchShape<DIM, K>;
+  constexpr int NP = PS::NP, L = PS::L, NPD = PS::NPD, NDC = PS::NDC, PPB = PS::PPB;
+  __shared__ CT buf[PPB][NPD];
+  __shared__ CT A1[PPB][NPD];
+  __shared__ CT A2[PPB][DIM > 2 ? NPD : 1];
+  __shared__ CT Ss[PPB][DIM][K * K];
+  __shared__ CT ls[PPB][DIM][K];
+
+  const int pl = threadIdx.x / L, t = threadIdx.x % L;
+  const int j0 = patch_begin + int(blockIdx.x) * PPB;
+  const int j = j0 + pl;
+  const bool valid = pl < PPB && j < patch_end;
+  const int plc = pl < PPB ? pl : 0;
+  const bool owner = pl < PPB;
+
+  for(int idx = threadIdx.x; idx < PPB * DIM * K * K; idx += blockDim.x)
+  {
+    const int q = idx / (DIM * K * K), r = idx % (DIM * K * K);
+    const int d = r / (K * K), e = r % (K * K);
+    const int jj = j0 + q;
+    Ss[q][d][e] = jj < patch_end ? S_tab[size_t(fac[size_t(jj) * DIM + d]) * K * K + e] : CT(0);
+  }
+  for(int idx = threadIdx.x; idx < PPB * DIM * K; idx += blockDim.x)
+  {
+    const int q = idx / (DIM * K), r = idx % (DIM * K);
+    const int d = r / K, e = r % K;
+    const int jj = j0 + q;
+    ls[q][d][e] = jj < patch_end ? lam_tab[size_t(fac[size_t(jj) * DIM + d]) * K + e] : CT(1);
+  }
+  int v[3] = {1, 1, 1};
+  if(valid) unpack_vertex(vtx[j], v);
+  const CT w = valid ? (omega ? CT(omega[j]) : relaxation) : CT(0);
+
+  const int ty = DIM > 1 ? t % K : 0;
+  const int tz = DIM > 2 ? t / K : 0;
+  const int ly = ty >= NP ? ty - NP : ty, lz = tz >= NP ? tz - NP : tz;
+  const int by = ty >= NP ? 1 : 0, bz = tz >= NP ? 1 : 0;
+  const int64_t nn = int64_t(n) * n;
+
+  CT g[K], r[K], o[K];
+#pragma unroll
+  for(int i = 0; i < K; ++i)
+    g[i] = valid ? CT(b[patch_cell<DIM, NP>(v, n, i, ty, tz) * NDC + patch_local<DIM, NP>(i, ty, tz)])
+                 : CT(0);
+  __syncthreads();
+
+  if(!x_is_zero)
+  {
+    // z_0: x-neighbours of the two x-cells of this line
+    CT z0[K], t1[K], t2[K];
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+    {
+      z0[i] = 0;
+      t1[i] = 0;
+      t2[i] = 0;
+    }
+    const int64_t c_lo = patch_cell<DIM, NP>(v, n, 0, ty, tz);      // x-cell with b_0 = 0
+    const int lyz = patch_local<DIM, NP>(0, ty, tz);                 // local offset of the line
+    if(valid && v[0] - 1 > 0)
+    {
+      const VT *xl = x + (c_lo - 1) * NDC + lyz;
+#pragma unroll
+      for(int i = 0; i < NP; ++i)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int jj = 0; jj < NP; ++jj) s += tab.Fpm[i * NP + jj] * CT(xl[jj]);
+        z0[i] = s;
+      }
+    }
+    if(valid && v[0] + 1 < n)
+    {
+      const VT *xr = x + (c_lo + 2) * NDC + lyz;
+#pragma unroll
+      for(int i = 0; i < NP; ++i)
+      {
+        CT s = 0;
+#pragma unroll
+        for(int jj = 0; jj < NP; ++jj) s += tab.Fmp[i * NP + jj] * CT(xr[jj]);
+        z0[NP + i] = s;
+      }
+    }
+    // z_1 (y-neighbour outside the patch) then t1 = M_x z_1 per x-cell
+    {
+      const bool has = by == 0 ? (v[1] - 1 > 0) : (v[1] + 1 < n);
+      const CT *F = by == 0 ? tab.Fpm : tab.Fmp;
+      if(valid && has)
+      {
+        const int64_t shift = by == 0 ? -int64_t(n) : int64_t(n);
+        CT z1[K];
+#pragma unroll
+        for(int cx = 0; cx < 2; ++cx)
+        {
+          const VT *xn = x + (c_lo + cx + shift) * NDC + NP * NP * lz;
+#pragma unroll
+          for(int lx = 0; lx < NP; ++lx)
+          {
+            CT s = 0;
+#pragma unroll
+            for(int jj = 0; jj < NP; ++jj) s += F[ly * NP + jj] * CT(xn[lx + NP * jj]);
+            z1[cx * NP + lx] = s;
+          }
+        }
+#pragma unroll
+        for(int cx = 0; cx < 2; ++cx)
+#pragma unroll
+          for(int i = 0; i < NP; ++i)
+          {
+            CT s = 0;
+#pragma unroll
+            for(int jj = 0; jj < NP; ++jj) s += tab.M[i * NP + jj] * z1[cx * NP + jj];
+            t1[cx * NP + i] = s;
+          }
+      }
+    }
+    if(DIM > 2)
+    {
+      const bool has = bz == 0 ? (v[2] - 1 > 0) : (v[2] + 1 < n);
+      const CT *F = bz == 0 ? tab.Fpm : tab.Fmp;
+      if(valid && has)
+      {
+        const int64_t shift = bz == 0 ? -nn : nn;
+        CT z2[K];
+#pragma unroll
+        for(int cx = 0; cx < 2; ++cx)
+        {
+          const VT *xn = x + (c_lo + cx + shift) * NDC + NP * ly;
+#pragma unroll
+          for(int lx = 0; lx < NP; ++lx)
+          {
+            CT s = 0;
+#pragma unroll
+            for(int kk = 0; kk < NP; ++kk) s += F[lz * NP + kk] * CT(xn[lx + NP * NP * kk]);
+            z2[cx * NP + lx] = s;
+          }
+        }
+#pragma unroll
+        for(int cx = 0; cx < 2; ++cx)
+#pragma unroll
+          for(int i = 0; i < NP; ++i)
+          {
+            CT s = 0;
+#pragma unroll
+            for(int jj = 0; jj < NP; ++jj) s += tab.M[i * NP + jj] * z2[cx * NP + jj];
+            t2[cx * NP + i] = s;
+          }
+      }
+    }
+    // cross-line mass contractions through LDS
+    CT *Z = A1[plc];
+    CT *T2 = A2[plc];
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+      if(owner)
+      {
+        Z[i + K * ty + K * K * tz] = z0[i];
+        if(DIM > 2) T2[i + K * ty + K * K * tz] = t2[i];
+      }
+    __syncthreads();
+    CT a[K], bb[K];
+    const int y0 = by * NP;
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+    {
+      CT s = t1[i], s2 = 0;
+#pragma unroll
+      for(int jj = 0; jj < NP; ++jj)
+      {
+        s += tab.M[ly * NP + jj] * Z[i + K * (y0 + jj) + K * K * tz];
+        if(DIM > 2) s2 += tab.M[ly * NP + jj] * T2[i + K * (y0 + jj) + K * K * tz];
+      }
+      a[i] = s;
+      bb[i] = s2;
+    }
+    if(DIM == 2)
+    {
+#pragma unroll
+      for(int i = 0; i < K; ++i) g[i] -= a[i];
+    }
+    else
+    {
+      __syncthreads();
+#pragma unroll
+      for(int i = 0; i < K; ++i)
+        if(owner) Z[i + K * ty + K * K * tz] = a[i];
+      __syncthreads();
+      const int z00 = bz * NP;
+#pragma unroll
+      for(int i = 0; i < K; ++i)
+      {
+        CT s = bb[i];
+#pragma unroll
+        for(int kk = 0; kk < NP; ++kk) s += tab.M[lz * NP + kk] * Z[i + K * ty + K * K * (z00 + kk)];
+        g[i] -= s;
+      }
+    }
+  }
+
+#pragma unroll
+  for(int i = 0; i < K; ++i) r[i] = g[i];
+  fd_lines<DIM, K, CT>(r, o, buf[plc], Ss[plc][0], Ss[plc][1], Ss[plc][DIM > 2 ? 2 : 1],
+                       ls[plc][0], ls[plc][1], ls[plc][DIM > 2 ? 2 : 1], w, t, owner);
+  if(!valid) return;
+  const bool keep = !x_is_zero && w != CT(1);
+#pragma unroll
+  for(int a = 0; a < K; ++a)
+  {
+    VT *p = x + patch_cell<DIM, NP>(v, n, a, ty, tz) * NDC + patch_local<DIM, NP>(a, ty, tz);
+    *p = keep ? VT((CT(1) - w) * CT(*p) + o[a]) : VT(o[a]);
+  }
+}
+
+}  // namespace tpss

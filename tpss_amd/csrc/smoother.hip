@@ -18,6 +18,7 @@
 #include "kernels.hpp"
 #include "ops.hpp"
 #include "setup_kernels.hpp"
+#include "kernels_mvp.hpp"
 #include "smoother.hpp"
 
 namespace tpss
@@ -281,6 +282,38 @@ void patch_pass(tpss_smoother *sm, const VT *src, VT *dst, CT *ubuf, int begin, 
   });
 }
 
+// residual-free multiplicative color pass (Cartesian mesh, red-black colors)
+template<typename VT, typename CT>
+void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, bool x_is_zero,
+                    hipStream_t s)
+{
+  const LocalSolvers &ls = *sm->ls;
+  if(end <= begin) return;
+  const CT *S = std::is_same<CT, double>::value ? (const CT *)ls.S64.ptr : (const CT *)ls.S32.ptr;
+  const CT *lam =
+    std::is_same<CT, double>::value ? (const CT *)ls.lam64.ptr : (const CT *)ls.lam32.ptr;
+  const float *om = sm->use_omega ? sm->d_omega.ptr : nullptr;
+  const CT relax = CT(sm->desc.relaxation);
+  dispatch_dim_np(ls.dim, ls.np, [&](auto D, auto NPc) {
+    constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value, K = 2 * NP;
+    using PS = PatchShape<DIM, K>;
+    OpTab<CT, NP> tab;
+    const Operator1D &op = sm->level->op1d;
+    for(int e = 0; e < NP * NP; ++e)
+    {
+      tab.M[e] = CT(op.M[e]);
+      tab.Fpm[e] = CT(op.Fpm[e]);
+      tab.Fmp[e] = CT(op.Fmp[e]);
+      for(int c = 0; c < 4; ++c) tab.D[c][e] = CT(op.D[c][e]);
+    }
+    const unsigned grid = unsigned((end - begin + PS::PPB - 1) / PS::PPB);
+    hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT>), dim3(grid), dim3(PS::THREADS), 0, s, b,
+                       x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end, ls.n,
+                       x_is_zero ? 1 : 0);
+    TPSS_HIP(hipGetLastError());
+  });
+}
+
 template<typename VT, typename CT>
 void avp_sum(tpss_smoother *sm, const CT *ubuf, VT *dst, bool accumulate, double alpha,
              hipStream_t s)
@@ -369,6 +402,19 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
   {
     sm->tmp2.resize(size_t(lv->N) * sizeof(VT));
     out = (VT *)sm->tmp2.ptr;
+  }
+  if(!lv->distorted && ls.coloring == TPSS_COLOR_REDBLACK && sm->fused_mvp)
+  {
+    // cells not yet reached by a color are read as neighbours: start from zero
+    fill_zero(out, lv->vec_prec, lv->N, s);
+    for(size_t k = 0; k < sm->seq.size(); ++k)
+    {
+      const int c = sm->seq[k];
+      SectionTimer t(sm, 1, s);
+      mvp_fused_pass<VT, CT>(sm, src, out, ls.color_ptr[c], ls.color_ptr[c + 1], k == 0, s);
+    }
+    if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
+    return;
   }
   sm->res.resize(size_t(lv->N) * sizeof(VT));
   VT *res = (VT *)sm->res.ptr;
@@ -650,6 +696,14 @@ int tpss_smoother_set_timing(tpss_smoother *sm, int enabled)
       for(int i = 0; i < 5; ++i)
         if(i != 2) sm->timers[i] = Timer{};
     sm->timing = enabled != 0;
+  });
+}
+
+int tpss_smoother_set_fused_residual(tpss_smoother *sm, int enabled)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm, "null argument");
+    sm->fused_mvp = enabled != 0;
   });
 }
 

@@ -47,6 +47,7 @@ struct tpss_smoother
   bool use_omega = false;
   bool timing = false;
   bool check_finite = false;
+  bool fused_mvp = true;  // residual-free color passes where the mesh allows them
   tpss::Timer timers[5];
   struct PendingSection
   {

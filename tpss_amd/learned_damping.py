@@ -1,0 +1,112 @@
+"""Learned per-patch damping: training-data generation, training and use in GMG.
+
+Training data: on seeded distorted meshes (interior vertices moved by
+amplitude*h*U[-1,1]^dim) every vertex patch contributes its features (the
+singular values of its cell Jacobians) and the optimal local damping that the
+smoother computes for it with its own local solvers and operator
+(SchwarzPreconditioner.optimal_damping).  The network is trained on the
+device by the library's own forward/backward/Adam kernels.
+
+    python -m tpss_amd.learned_damping --out tpss_amd/models/damping_dgq3_3d.npy
+"""
+import argparse
+import os
+import time
+
+import numpy as np
+
+from . import (F64, MULTIPLICATIVE, COLOR_REDBLACK, DampingMLP, Level, SchwarzSmoother)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_MODEL = os.path.join(HERE, "models", "damping_dgq3_3d.npy")
+
+
+def distorted_vertices(dim, n, amplitude, seed):
+    """Interior vertices moved by (amplitude/n) * U[-1,1]^dim; U from SplitMix64
+    keyed by (seed, vertex*dim + d); boundary vertices fixed."""
+    g = np.linspace(0.0, 1.0, n + 1)
+    mesh = np.meshgrid(*([g] * dim), indexing="ij")
+    v = np.stack([m.transpose(tuple(range(dim))[::-1]).ravel() for m in mesh], axis=-1)
+    nv = v.shape[0]
+    i = np.arange(nv * dim, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) * np.uint64(0x100000000) + i + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    noise = (2.0 * ((z >> np.uint64(11)).astype(np.float64) / 9007199254740992.0) - 1.0).reshape(nv, dim)
+    idx = np.arange(nv)
+    interior = np.ones(nv, dtype=bool)
+    t = idx.copy()
+    for _ in range(dim):
+        c = t % (n + 1)
+        interior &= (c > 0) & (c < n)
+        t //= n + 1
+    v[interior] += (amplitude / n) * noise[interior]
+    return v
+
+
+def patch_samples(dim, degree, n, amplitude, seed, ip_factor=4.0, n_samples=4,
+                  omega_min=0.2, omega_max=1.2):
+    """(features, optimal damping) of every patch of one distorted level."""
+    V = distorted_vertices(dim, n, amplitude, seed)
+    lv = Level(dim, degree, n, ip_factor=ip_factor, vertices=V, precision=F64)
+    sm = SchwarzSmoother(lv, variant=MULTIPLICATIVE, coloring=COLOR_REDBLACK, relaxation=1.0)
+    X = sm.patch_features()
+    y = sm.optimal_damping(n_samples=n_samples, seed=1000 + seed, omega_min=omega_min,
+                           omega_max=omega_max)
+    return X, y
+
+
+def generate(dim=3, degree=3, n=32, amplitude=0.2, seeds=range(101, 109), n_samples=4):
+    Xs, ys = [], []
+    for s in seeds:
+        X, y = patch_samples(dim, degree, n, amplitude, s, n_samples=n_samples)
+        Xs.append(X)
+        ys.append(y)
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+def train(X, y, epochs=40, batch=2048, lr=2e-3, seed=4, hidden=64):
+    mlp = DampingMLP(X.shape[1], hidden, 0.2, 1.2, seed=seed)
+    loss0, _ = mlp.loss_grad(X[:8192], y[:8192])
+    loss = mlp.train(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return mlp, loss0, loss
+
+
+def load(path=DEFAULT_MODEL, n_in=24, hidden=64):
+    return DampingMLP(n_in, hidden, 0.2, 1.2).load(path)
+
+
+def apply_to_gmg(gmg, mlp):
+    """Per-patch damping from the network on every smoothed level (pre and post)."""
+    for level in range(1, gmg.n_levels):
+        for post in (False, True):
+            gmg.smoother(level, post).set_damping_mlp(mlp)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_MODEL)
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--degree", type=int, default=3)
+    ap.add_argument("--amplitude", type=float, default=0.2)
+    ap.add_argument("--n-meshes", type=int, default=7)
+    ap.add_argument("--epochs", type=int, default=40)
+    args = ap.parse_args()
+    t0 = time.time()
+    X, y = generate(3, args.degree, args.n, args.amplitude, seeds=range(101, 101 + args.n_meshes))
+    t1 = time.time()
+    print(f"training data: {X.shape[0]} patches from {args.n_meshes} meshes in {t1 - t0:.1f} s; "
+          f"target mean {y.mean():.4f} std {y.std():.4f} min {y.min():.3f} max {y.max():.3f}",
+          flush=True)
+    mlp, l0, l1 = train(X, y, epochs=args.epochs)
+    print(f"training: loss {l0:.3e} -> {l1:.3e} in {time.time() - t1:.1f} s "
+          f"(variance of the target {0.5 * y.var():.3e})", flush=True)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    mlp.save(args.out)
+    print("saved", args.out, flush=True)
+
+
+if __name__ == "__main__":
+    main()
