@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gmg", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the distorted-mesh GMG-CG with learned vs fixed damping")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=2)
     return ap.parse_args()
@@ -113,16 +115,19 @@ def main():
     n, p, dim = args.n, args.degree, 3
     ndc = (p + 1) ** dim
     results = {}
-    for name, prec in (("f64", t.F64), ("f32", t.F32)):
+    for name, prec, lprec in (("f64", t.F64, t.F64), ("f32", t.F32, t.F32),
+                              ("bf16", t.F32, t.BF16)):
         lv = t.Level(dim, p, n, precision=prec)
         sm = t.SchwarzSmoother(lv, variant=t.MULTIPLICATIVE, coloring=t.COLOR_REDBLACK,
-                               relaxation=1.0, local_precision=prec)
+                               relaxation=1.0, local_precision=lprec)
         sec, sections = time_smoother(t, lv, sm, args.steps, args.warmup, prec)
         s_v = 8 if prec == t.F64 else 4
         pd = sm.n_patches * sm.n_dofs_per_patch
         apply_t, apply_n = sections["[SchwarzPrecond] Apply inverses:"]
         res_t, res_n = sections["[SchwarzPrecond] Update residuals:"]
-        patch_bytes = sm.n_patches / sm.n_colors * 3 * sm.n_dofs_per_patch * s_v
+        # residual-free color pass: read b_j, the 24 outer neighbour cells (3 n_pd
+        # values) and write x_j per patch
+        patch_bytes = sm.n_patches / sm.n_colors * 5 * sm.n_dofs_per_patch * s_v
         res_bytes = residual_kernel_bytes(sm, n, ndc, s_v)
         kernels = {
             "patch_fd": {"avg_ms": 1e3 * apply_t / max(apply_n, 1), "launches": apply_n,
@@ -140,9 +145,11 @@ def main():
 
     gmg = {}
     if not args.no_gmg:
-        for name, mp in (("f64", t.F64), ("mixed_f32", t.F32)):
+        for name, mp, lp in (("f64", t.F64, t.F64), ("mixed_f32", t.F32, t.F32),
+                             ("mixed_bf16", t.F32, t.BF16)):
             g = t.GMGSolver(dim, p, 2, int(np.log2(n // 2)) + 1, mg_precision=mp,
-                            variant=t.MULTIPLICATIVE, coloring=t.COLOR_REDBLACK, relaxation=1.0)
+                            variant=t.MULTIPLICATIVE, coloring=t.COLOR_REDBLACK, relaxation=1.0,
+                            local_precision=lp)
             b = t.DeviceVector(g.n_dofs)
             t.fill_random(b, 2)
             x = t.DeviceVector.zeros(g.n_dofs)
@@ -151,6 +158,30 @@ def main():
             gmg[name] = {"seconds": info.seconds, "iterations": info.iterations,
                          "n_frac": info.n_frac, "reduction_rate": info.reduction_rate,
                          "final_rel_residual": hist[-1] / hist[0]}
+            del g
+
+    learned = {}
+    model = os.path.join(ROOT, "tpss_amd", "models", "damping_dgq3_3d.npy")
+    if not args.no_gmg and not args.no_learned and os.path.exists(model):
+        # learned damping: same level with interior vertices moved by 0.2 h (seed 3),
+        # ip_factor x4 as the reference does on distorted meshes
+        from tpss_amd import learned_damping as ld
+        V = ld.distorted_vertices(dim, n, 0.2, seed=3)
+        mlp = ld.load(model)
+        for name, mp, use in (("f64_fixed", t.F64, False), ("f64_learned", t.F64, True),
+                              ("mixed_f32_fixed", t.F32, False), ("mixed_f32_learned", t.F32, True)):
+            g = t.GMGSolver(dim, p, 2, int(np.log2(n // 2)) + 1, ip_factor=4.0, vertices=V,
+                            mg_precision=mp, variant=t.MULTIPLICATIVE,
+                            coloring=t.COLOR_REDBLACK, relaxation=1.0)
+            if use:
+                ld.apply_to_gmg(g, mlp)
+            b = t.DeviceVector(g.n_dofs)
+            t.fill_random(b, 2)
+            x = t.DeviceVector.zeros(g.n_dofs)
+            info, hist = g.solve(x, b, rel_tol=1e-8, max_iter=100, raise_on_fail=False)
+            learned[name] = {"seconds": info.seconds, "iterations": info.iterations,
+                             "n_frac": info.n_frac, "reduction_rate": info.reduction_rate,
+                             "final_rel_residual": hist[-1] / hist[0]}
             del g
 
     cpu = None
@@ -188,6 +219,7 @@ def main():
                             "ms_per_step": r["ms_per_step"],
                             "kernels": r["kernels"]} for name, r in results.items()},
         "gmg_cg": gmg,
+        "gmg_cg_distorted_learned_damping": learned,
     }
     print(json.dumps(out))
 

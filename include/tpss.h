@@ -205,6 +205,14 @@ int tpss_smoother_optimal_damping(tpss_smoother *sm, int n_samples, uint64_t see
 /* omega_j = MLP(features_j) for every patch (NULL restores the scalar relaxation) */
 int tpss_smoother_set_damping_mlp(tpss_smoother *sm, tpss_mlp *m);
 
+/* ---- Kronecker SVD --------------------------------------------------------- */
+/* compute_ksvd (kroneckersvd.h:215-296) for a batch of matrices, FP64 on the
+ * device: A (device, batch x (m1 m0) x (n1 n0), row-major) ~ sum_{i<rank}
+ * B1_i (x) B0_i; B1: batch x rank x m1 x n1, B0: batch x rank x m0 x n0,
+ * sigma: batch x rank (device). */
+int tpss_ksvd_batched(const double *A, int batch, int m1, int n1, int m0, int n0, int rank,
+                      double *B1, double *B0, double *sigma, void *stream);
+
 /* ---- grid transfer (MGTransferMatrixFree for FE_DGQ) --------------------- */
 int tpss_transfer_prolongate_add(tpss_level *coarse, tpss_level *fine, void *dst_fine,
                                  const void *src_coarse, void *stream);

@@ -83,6 +83,7 @@ EXPORTS = [
     "tpss_mlp_create", "tpss_mlp_destroy", "tpss_mlp_n_params", "tpss_mlp_set_params",
     "tpss_mlp_get_params", "tpss_mlp_forward", "tpss_mlp_loss_grad", "tpss_mlp_train",
     "tpss_smoother_patch_features", "tpss_smoother_optimal_damping", "tpss_smoother_set_damping_mlp",
+    "tpss_ksvd_batched",
 ]
 
 _VP = ctypes.c_void_p
@@ -153,6 +154,9 @@ def _load():
         "tpss_smoother_optimal_damping": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64,
                                                          ctypes.c_double, ctypes.c_double, _VP]),
         "tpss_smoother_set_damping_mlp": (ctypes.c_int, [_VP, _VP]),
+        "tpss_ksvd_batched": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP,
+                                             _VP, _VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -569,6 +573,27 @@ class DampingMLP:
         if getattr(self, "handle", None):
             lib.tpss_mlp_destroy(self.handle)
             self.handle = None
+
+
+def ksvd(A, m1, n1, m0, n0, rank):
+    """Batched Kronecker SVD on the device: A (batch, m1*m0, n1*n0) or a single
+    matrix -> (B1 (batch, rank, m1, n1), B0 (batch, rank, m0, n0), sigma)."""
+    import numpy as np
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    single = A.ndim == 2
+    if single:
+        A = A[None]
+    batch = A.shape[0]
+    dA = DeviceVector.from_numpy(A.ravel(), F64)
+    dB1 = DeviceVector(batch * rank * m1 * n1)
+    dB0 = DeviceVector(batch * rank * m0 * n0)
+    ds = DeviceVector(batch * rank)
+    _check(lib.tpss_ksvd_batched(_ptr(dA), batch, m1, n1, m0, n0, rank, _ptr(dB1), _ptr(dB0),
+                                 _ptr(ds), None))
+    B1 = dB1.numpy().reshape(batch, rank, m1, n1)
+    B0 = dB0.numpy().reshape(batch, rank, m0, n0)
+    s = ds.numpy().reshape(batch, rank)
+    return (B1[0], B0[0], s[0]) if single else (B1, B0, s)
 
 
 def version():

@@ -21,7 +21,81 @@
 
 namespace tpss
 {
-template<int DIM, int K, typename VT, typename CT>
+typedef short tpss_bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
+typedef float tpss_floatx4 __attribute__((__vector_size__(4 * sizeof(float))));
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ short bf16_bits(float f)
+{
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return short(u >> 16);
+}
+
+// Fast-diagonalization inverse of the two patches of a block (3D, K = 8) on
+// the matrix cores: every 1D contraction along direction d is one
+// v_mfma_f32_16x16x16_bf16 per chunk of 16 lines, with A = diag(S_a^T, S_b^T)
+// (16 x 16, block diagonal over the two patches) and B the 16 contraction
+// values (8 per patch) of 16 lines; products in BF16, sums in FP32.  The
+// patch data (FP32) lives in B2[2][512] (x fastest); the two waves of the
+// block take two chunks each.  Lane layout of the 16x16x16 MFMA (wave64):
+// A: row l%16, k = 4(l/16)+0..3; B: column l%16, k = 4(l/16)+0..3;
+// D: column l%16, rows 4(l/16)+0..3.
+__device__ __forceinline__ int lds_index(int d, int k, int line)
+{
+  // K = 8: line enumerates the other two indices (first fastest)
+  return d == 0 ? k + 8 * line : d == 1 ? (line & 7) + 8 * k + 64 * (line >> 3) : line + 64 * k;
+}
+
+__device__ __forceinline__ void fd_mfma_pair(float (*B2)[512], const float (*S)[3][64],
+                                             const float (*lam)[3][8], const float *w2)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n16 = lane & 15, q = lane >> 4;       // column (line) / k-quarter
+  const int pq = q >> 1, kofs = 4 * (q & 1);      // patch of the B/D rows, first row index
+  const int arow = lane & 15, pa = arow >> 3, ao = arow & 7;  // A row: patch, output index
+  for(int stage = 0; stage < 6; ++stage)
+  {
+    const int d = stage < 3 ? stage : 5 - stage;
+    const bool fwd = stage < 3;
+    // A operand: S_p^T (forward) or S_p (backward) of patch pa if the k block
+    // belongs to the same patch, else zero
+    tpss_bf16x4 a;
+#pragma unroll
+    for(int e = 0; e < 4; ++e)
+    {
+      const int kk = kofs + e;
+      float v = 0.f;
+      if(pa == pq) v = fwd ? S[pa][d][kk * 8 + ao] : S[pa][d][ao * 8 + kk];
+      a[e] = bf16_bits(v);
+    }
+#pragma unroll
+    for(int cc = 0; cc < 2; ++cc)
+    {
+      const int line = 16 * (2 * wave + cc) + n16;
+      tpss_bf16x4 b;
+#pragma unroll
+      for(int e = 0; e < 4; ++e) b[e] = bf16_bits(B2[pq][lds_index(d, kofs + e, line)]);
+      tpss_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      if(stage == 2)
+      {
+        // diagonal scaling after the last forward contraction (z): line = (x, y)
+        const int x0 = line & 7, y0 = line >> 3;
+        const float lxy = lam[pq][0][x0] + lam[pq][1][y0];
+#pragma unroll
+        for(int e = 0; e < 4; ++e) acc[e] *= w2[pq] / (lxy + lam[pq][2][kofs + e]);
+      }
+#pragma unroll
+      for(int e = 0; e < 4; ++e) B2[pq][lds_index(d, kofs + e, line)] = acc[e];
+    }
+    __syncthreads();
+  }
+}
+
+// FD = 0: vector FMA contractions in CT; FD = 1: BF16 MFMA contractions with
+// FP32 accumulation (3D, K = 8, CT = float)
+template<int DIM, int K, typename VT, typename CT, int FD = 0>
 __global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS))
 k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, K / 2> tab,
                   const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
@@ -228,10 +302,29 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
     }
   }
 
+  if constexpr(FD == 1)
+  {
+    static_assert(DIM == 3 && K == 8 && PPB == 2, "MFMA patch solve: 3D, K = 8, two patches per block");
+    __shared__ float w2[2];
+    if(t == 0 && owner) w2[pl] = float(w);
+    __syncthreads();  // the coupling stage above may still read A1/A2; buf is free
 #pragma unroll
-  for(int i = 0; i < K; ++i) r[i] = g[i];
-  fd_lines<DIM, K, CT>(r, o, buf[plc], Ss[plc][0], Ss[plc][1], Ss[plc][DIM > 2 ? 2 : 1],
-                       ls[plc][0], ls[plc][1], ls[plc][DIM > 2 ? 2 : 1], w, t, owner);
+    for(int i = 0; i < K; ++i)
+      if(owner) buf[plc][i + K * ty + K * K * tz] = float(g[i]);
+    __syncthreads();
+    fd_mfma_pair(reinterpret_cast<float(*)[512]>(&buf[0][0]),
+                 reinterpret_cast<const float(*)[3][64]>(&Ss[0][0][0]),
+                 reinterpret_cast<const float(*)[3][8]>(&ls[0][0][0]), w2);
+#pragma unroll
+    for(int i = 0; i < K; ++i) o[i] = buf[plc][i + K * ty + K * K * tz];
+  }
+  else
+  {
+#pragma unroll
+    for(int i = 0; i < K; ++i) r[i] = g[i];
+    fd_lines<DIM, K, CT>(r, o, buf[plc], Ss[plc][0], Ss[plc][1], Ss[plc][DIM > 2 ? 2 : 1],
+                         ls[plc][0], ls[plc][1], ls[plc][DIM > 2 ? 2 : 1], w, t, owner);
+  }
   if(!valid) return;
   const bool keep = !x_is_zero && w != CT(1);
 #pragma unroll

@@ -282,8 +282,9 @@ void patch_pass(tpss_smoother *sm, const VT *src, VT *dst, CT *ubuf, int begin, 
   });
 }
 
-// residual-free multiplicative color pass (Cartesian mesh, red-black colors)
-template<typename VT, typename CT>
+// residual-free multiplicative color pass (Cartesian mesh, red-black colors);
+// FD = 1 selects the BF16-MFMA patch solve (3D, Q3)
+template<typename VT, typename CT, int FD = 0>
 void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, bool x_is_zero,
                     hipStream_t s)
 {
@@ -307,9 +308,19 @@ void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, b
       for(int c = 0; c < 4; ++c) tab.D[c][e] = CT(op.D[c][e]);
     }
     const unsigned grid = unsigned((end - begin + PS::PPB - 1) / PS::PPB);
-    hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT>), dim3(grid), dim3(PS::THREADS), 0, s, b,
-                       x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end, ls.n,
-                       x_is_zero ? 1 : 0);
+    if constexpr(FD == 1)
+    {
+      if constexpr(DIM == 3 && K == 8 && std::is_same<CT, float>::value)
+        hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT, 1>), dim3(grid), dim3(PS::THREADS), 0,
+                           s, b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end,
+                           ls.n, x_is_zero ? 1 : 0);
+      else
+        throw Error(TPSS_E_UNSUPPORTED, "BF16-MFMA patch solves are built for 3D Q3 (k = 8)");
+    }
+    else
+      hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT>), dim3(grid), dim3(PS::THREADS), 0, s,
+                         b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end, ls.n,
+                         x_is_zero ? 1 : 0);
     TPSS_HIP(hipGetLastError());
   });
 }
@@ -434,6 +445,39 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
   }
   if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
 }
+
+// BF16-MFMA patch solves (FP32 accumulation) for the multiplicative variant on
+// Cartesian meshes with red-black colors, 3D Q3; vectors FP32 or FP64
+template<typename VT>
+void precond_vmult_bf16(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
+                        double alpha)
+{
+  LocalSolvers &ls = *sm->ls;
+  const tpss_level *lv = sm->level;
+  if(sm->desc.variant != TPSS_MULTIPLICATIVE || lv->distorted ||
+     ls.coloring != TPSS_COLOR_REDBLACK || ls.dim != 3 || ls.K != 8)
+    throw Error(TPSS_E_UNSUPPORTED,
+                "BF16-MFMA patch solves: multiplicative, red-black, Cartesian 3D Q3 only");
+  SectionTimer step_timer(sm, 4, s);
+  VT *out = dst;
+  if(accumulate)
+  {
+    sm->tmp2.resize(size_t(lv->N) * sizeof(VT));
+    out = (VT *)sm->tmp2.ptr;
+  }
+  fill_zero(out, lv->vec_prec, lv->N, s);
+  for(size_t k = 0; k < sm->seq.size(); ++k)
+  {
+    const int c = sm->seq[k];
+    SectionTimer t(sm, 1, s);
+    mvp_fused_pass<VT, float, 1>(sm, src, out, ls.color_ptr[c], ls.color_ptr[c + 1], k == 0, s);
+  }
+  if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
+}
+template void precond_vmult_bf16<double>(tpss_smoother *, double *, const double *, hipStream_t,
+                                         bool, double);
+template void precond_vmult_bf16<float>(tpss_smoother *, float *, const float *, hipStream_t, bool,
+                                        double);
 
 template<typename VT>
 void precond_dispatch(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
