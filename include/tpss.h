@@ -38,6 +38,12 @@ typedef enum
 typedef enum { TPSS_ADDITIVE = 0, TPSS_MULTIPLICATIVE = 1 } tpss_smoother_variant;
 typedef enum { TPSS_COLOR_NONE = 0, TPSS_COLOR_TILED = 1, TPSS_COLOR_REDBLACK = 2 } tpss_coloring;
 typedef enum { TPSS_F64 = 0, TPSS_F32 = 1, TPSS_BF16 = 2 } tpss_precision;
+/* TPSS_ELASTICITY: SIPG linear elasticity with dim components
+ * (LinElasticity::MW::MatrixIntegrator, apps/linelasticity_integrator.h:31-124),
+ * component-blocked vectors like the reference's block vectors, Cartesian
+ * meshes; the Schwarz smoother uses the block-diagonal local solvers
+ * (FD::MatrixIntegrator::assemble_subspace_inverses, :604-647). */
+typedef enum { TPSS_POISSON = 0, TPSS_ELASTICITY = 1 } tpss_equation;
 
 typedef struct tpss_level tpss_level;
 typedef struct tpss_smoother tpss_smoother;
@@ -55,6 +61,9 @@ typedef struct
   const double *vertices; /* host, (n+1)^dim x dim coordinates, x fastest;
                              NULL = uniform Cartesian mesh */
   int vector_precision;   /* TPSS_F64 or TPSS_F32: element type of level vectors */
+  int equation;           /* tpss_equation (0 = Poisson) */
+  double mu, lambda;      /* Lame coefficients (TPSS_ELASTICITY, EquationData
+                             apps/equation_data.h:382-385) */
 } tpss_level_desc;
 
 /* Smoother settings: SchwarzPreconditioner::AdditionalData
@@ -155,6 +164,8 @@ typedef struct
   int mg_precision;        /* TPSS_F64 or TPSS_F32: V-cycle vector precision */
   tpss_smoother_desc smoother; /* pre-smoother; post-smoother = reversed colors */
   double coarse_tol;       /* relative accuracy of the FP64 coarse CG */
+  int equation;            /* tpss_equation (0 = Poisson) */
+  double mu, lambda;       /* Lame coefficients (TPSS_ELASTICITY) */
 } tpss_gmg_desc;
 
 typedef struct

@@ -92,7 +92,15 @@ public:
   Level(int dim, int degree, int n_cells_1d, double ip_factor = 1.0,
         const double *vertices = nullptr)
   {
-    tpss_level_desc d{dim, degree, n_cells_1d, ip_factor, vertices, precision_of<Number>::value};
+    tpss_level_desc d{dim, degree, n_cells_1d, ip_factor, vertices, precision_of<Number>::value,
+                      TPSS_POISSON, 0.0, 0.0};
+    check(tpss_level_create(&d, &h_));
+  }
+  // linear elasticity with Lame coefficients mu, lambda (LinElasticity::ModelProblem)
+  Level(int dim, int degree, int n_cells_1d, double ip_factor, double mu, double lambda)
+  {
+    tpss_level_desc d{dim, degree, n_cells_1d, ip_factor, nullptr, precision_of<Number>::value,
+                      TPSS_ELASTICITY, mu, lambda};
     check(tpss_level_create(&d, &h_));
   }
   Level(const Level &) = delete;

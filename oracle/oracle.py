@@ -35,11 +35,20 @@ def _load():
         "or_penalty": (ctypes.c_double, [ctypes.c_int, ctypes.c_double, ctypes.c_double]),
         "or_patch_matrices_1d": (None, [ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_double, _D, _D]),
+        "or_patch_matrices_1d_coef": (None, [ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             _D, _D]),
         "or_gen_eig": (ctypes.c_int, [ctypes.c_int, _D, _D, _D, _D]),
         "or_fd_apply": (None, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]),
         "or_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+        "or_create_elasticity": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_double, ctypes.c_int]),
+        "or_n_components": (ctypes.c_int, [ctypes.c_void_p]),
         "or_destroy": (None, [ctypes.c_void_p]),
         "or_set_vertices": (ctypes.c_int, [ctypes.c_void_p, _D]),
         "or_setup": (ctypes.c_int, [ctypes.c_void_p]),
@@ -68,6 +77,11 @@ def _load():
         "or_gmg_create_v": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]),
+        "or_gmg_create_elasticity": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                       ctypes.c_int, ctypes.c_double,
+                                                       ctypes.c_double, ctypes.c_double,
+                                                       ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                       ctypes.c_int, ctypes.c_double]),
         "or_gmg_destroy": (None, [ctypes.c_void_p]),
         "or_gmg_level": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int]),
         "or_gmg_vcycle": (None, [ctypes.c_void_p, _D, _D]),
@@ -119,6 +133,15 @@ def patch_matrices_1d(p, h, bdry_left, bdry_right, ip):
     h = np.ascontiguousarray(h, dtype=np.float64)
     M, L = np.zeros((k, k)), np.zeros((k, k))
     lib.or_patch_matrices_1d(p, dp(h), int(bdry_left), int(bdry_right), float(ip), dp(M), dp(L))
+    return M, L
+
+
+def patch_matrices_1d_coef(p, h, bdry_left, bdry_right, ip, a_cons, a_pen):
+    k = 2 * (p + 1)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    M, L = np.zeros((k, k)), np.zeros((k, k))
+    lib.or_patch_matrices_1d_coef(p, dp(h), int(bdry_left), int(bdry_right), float(ip),
+                                  float(a_cons), float(a_pen), dp(M), dp(L))
     return M, L
 
 
@@ -178,10 +201,18 @@ class Smoother:
 
     def __init__(self, dim, p, n, ip=1.0, variant=ADDITIVE, coloring=COLOR_NONE,
                  relaxation=1.0, reverse=False, symmetrized=False, damping=1.0,
-                 n_steps=1, vertices=None, threads=1):
+                 n_steps=1, vertices=None, threads=1, elasticity=None):
+        """elasticity=(mu, lambda) selects the linear elasticity operator with
+        dim components and block-diagonal local solvers."""
         self.dim, self.p, self.n = dim, p, n
-        self.h = lib.or_create(dim, p, n, float(ip), variant, coloring, float(relaxation),
-                               int(reverse), int(symmetrized), float(damping), n_steps)
+        if elasticity is None:
+            self.h = lib.or_create(dim, p, n, float(ip), variant, coloring, float(relaxation),
+                                   int(reverse), int(symmetrized), float(damping), n_steps)
+        else:
+            mu, lam = elasticity
+            self.h = lib.or_create_elasticity(dim, p, n, float(ip), float(mu), float(lam), variant,
+                                              coloring, float(relaxation), int(reverse),
+                                              int(symmetrized), float(damping), n_steps)
         if not self.h:
             raise ValueError("invalid smoother parameters")
         if vertices is not None:
@@ -196,7 +227,8 @@ class Smoother:
         self.P = lib.or_n_patches(self.h)
         self.n_colors = lib.or_n_colors(self.h)
         self.k = 2 * (p + 1)
-        self.ndp = self.k ** dim
+        self.ncomp = lib.or_n_components(self.h)
+        self.ndp = self.ncomp * self.k ** dim  # patch-DoFs over all components
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -225,9 +257,11 @@ class Smoother:
         return out
 
     def patch_factors(self):
-        S = np.zeros((self.P, self.dim, self.k, self.k))
-        lam = np.zeros((self.P, self.dim, self.k))
+        S = np.zeros((self.P, self.ncomp, self.dim, self.k, self.k))
+        lam = np.zeros((self.P, self.ncomp, self.dim, self.k))
         lib.or_patch_factors(self.h, dp(S), dp(lam))
+        if self.ncomp == 1:
+            return S[:, 0], lam[:, 0]
         return S, lam
 
     def color_sequence(self, transpose=False):
@@ -279,14 +313,23 @@ class GMG:
 
     def __init__(self, dim, p, n_coarse, n_levels, ip=1.0, variant=MULTIPLICATIVE,
                  coloring=COLOR_REDBLACK, relaxation=1.0, n_steps=1, coarse_tol=1e-12,
-                 vertices=None):
+                 vertices=None, elasticity=None):
         vp = None
         if vertices is not None:
             self._v = np.ascontiguousarray(vertices, dtype=np.float64)
             vp = dp(self._v)
-        self.h = lib.or_gmg_create_v(dim, p, n_coarse, n_levels, float(ip), variant, coloring,
-                                     float(relaxation), n_steps, float(coarse_tol), vp)
-        self.N = (n_coarse << (n_levels - 1)) ** dim * (p + 1) ** dim
+        if elasticity is None:
+            self.h = lib.or_gmg_create_v(dim, p, n_coarse, n_levels, float(ip), variant, coloring,
+                                         float(relaxation), n_steps, float(coarse_tol), vp)
+            ncomp = 1
+        else:
+            assert vertices is None, "elasticity: Cartesian meshes only"
+            mu, lam = elasticity
+            self.h = lib.or_gmg_create_elasticity(dim, p, n_coarse, n_levels, float(ip), float(mu),
+                                                  float(lam), variant, coloring, float(relaxation),
+                                                  n_steps, float(coarse_tol))
+            ncomp = dim
+        self.N = ncomp * (n_coarse << (n_levels - 1)) ** dim * (p + 1) ** dim
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -163,6 +163,22 @@ double or_penalty(int p, double h_left, double h_right)
 void or_patch_matrices_1d(int p, const double *h, int bdry_left, int bdry_right,
                           double ip, double *M, double *L)
 {
+  or_patch_matrices_1d_coef(p, h, bdry_left, bdry_right, ip, 1.0, 1.0, M, L);
+}
+
+/*
+ * Same as or_patch_matrices_1d with L = a_cons * (cell stiffness + face
+ * consistency/symmetry terms) + a_pen * (face penalty terms).  The elasticity
+ * local solvers use this form: for component c in direction d the strain
+ * integrators scale the stiffness and consistency terms by
+ * 2 mu * (1 if c == d else 1/2) but the penalty by 2 mu
+ * (CellStrain / NitscheStrain, apps/linelasticity_integrator.h:753-779,
+ * 869-966, basic penalty variant), and grad-div adds lambda to both in
+ * direction c only (CellGradDiv / NitscheGradDiv, :839-867, 1059-1145).
+ */
+void or_patch_matrices_1d_coef(int p, const double *h, int bdry_left, int bdry_right,
+                               double ip, double a_cons, double a_pen, double *M, double *L)
+{
   const int np = p + 1, nq = p + 1, k = 2 * np;
   double nodes[OR_MAXNP], qx[OR_MAXNP], qw[OR_MAXNP];
   double val[OR_MAXNP * OR_MAXNP], grad[OR_MAXNP * OR_MAXNP];
@@ -185,7 +201,7 @@ void or_patch_matrices_1d(int p, const double *h, int bdry_left, int bdry_right,
           l += (grad[i * nq + q] / hc) * (grad[j * nq + q] / hc) * dx;
         }
         M[(c * np + i) * k + c * np + j] += m;
-        L[(c * np + i) * k + c * np + j] += l;
+        L[(c * np + i) * k + c * np + j] += a_cons * l;
       }
   }
 
@@ -205,8 +221,8 @@ void or_patch_matrices_1d(int p, const double *h, int bdry_left, int bdry_right,
         for(int j = 0; j < np; ++j)
         {
           const double uj = fval[j * 2 + f], guj = fgrad[j * 2 + f] / hc;
-          double v = -a * (vi * nrm * guj + gi * uj * nrm);
-          v += pen * vi * uj * nrm * nrm;
+          double v = -a_cons * a * (vi * nrm * guj + gi * uj * nrm);
+          v += a_pen * pen * vi * uj * nrm * nrm;
           L[(c * np + i) * k + c * np + j] += v;
         }
       }
@@ -225,10 +241,10 @@ void or_patch_matrices_1d(int p, const double *h, int bdry_left, int bdry_right,
       {
         const double u0 = fval[j * 2 + 1], gu0 = fgrad[j * 2 + 1] / h0;
         const double u1 = fval[j * 2 + 0], gu1 = fgrad[j * 2 + 0] / h1;
-        double v01 = -0.5 * (v0 * n0 * gu1 + gv0 * u1 * n1);
-        double v10 = -0.5 * (v1 * n1 * gu0 + gv1 * u0 * n0);
-        v01 += pen * v0 * u1 * n0 * n1;
-        v10 += pen * v1 * u0 * n1 * n0;
+        double v01 = -a_cons * 0.5 * (v0 * n0 * gu1 + gv0 * u1 * n1);
+        double v10 = -a_cons * 0.5 * (v1 * n1 * gu0 + gv1 * u0 * n0);
+        v01 += a_pen * pen * v0 * u1 * n0 * n1;
+        v10 += a_pen * pen * v1 * u0 * n1 * n0;
         L[(0 * np + i) * k + np + j] += v01;
         L[(np + i) * k + 0 * np + j] += v10;
       }
@@ -438,6 +454,8 @@ struct or_ctx
   int dim, p, n, np, k, ndc, ndp;
   long ncell, N;
   double ip;
+  int ncomp;         /* 1: Poisson; dim: linear elasticity (component-blocked vectors) */
+  double mu, lambda; /* Lame coefficients (elasticity) */
   int variant, coloring, reverse, symmetrized, n_steps, n_threads;
   double relaxation, rich_damping;
   double *vertices; /* (n+1)^dim * dim or NULL (Cartesian) */
@@ -447,7 +465,7 @@ struct or_ctx
   int *vtx;       /* P * dim */
   double *h;      /* P * dim * 2 */
   unsigned char *bdry; /* P * dim * 2 */
-  double *S, *lam;     /* P * dim * k * k, P * dim * k */
+  double *S, *lam;     /* P * ncomp * dim * k * k, P * ncomp * dim * k */
   double *omega;       /* P or NULL */
   /* operator data (Cartesian) */
   double Mc[OR_MAXNP * OR_MAXNP], Kc[OR_MAXNP * OR_MAXNP];
@@ -508,6 +526,7 @@ or_ctx *or_create(int dim, int p, int n, double ip, int variant, int coloring,
   c->ndp = ipow(c->k, dim);
   c->ncell = ipow(n, dim);
   c->N = c->ncell * c->ndc;
+  c->ncomp = 1;
   c->ip = ip;
   c->variant = variant;
   c->coloring = coloring;
@@ -520,6 +539,31 @@ or_ctx *or_create(int dim, int p, int n, double ip, int variant, int coloring,
   c->n_colors = coloring == OR_COLOR_NONE ? 1 : coloring == OR_COLOR_TILED ? (1 << dim) : 2 << dim;
   return c;
 }
+
+/*
+ * Linear elasticity (C5 of the reference apps): SIPG DG Q_p vector field with
+ * dim components on the Cartesian unit cube, Lame coefficients mu, lambda
+ * (apps/linelasticity_integrator.h:31-124 for the operator; block-diagonal
+ * local solvers :604-647).  Vectors are component-blocked like the
+ * reference's block vectors (one scalar DG block per component,
+ * patch_transfer.h:285-418): x[c * ncell * (p+1)^dim + cell * (p+1)^dim + local].
+ */
+or_ctx *or_create_elasticity(int dim, int p, int n, double ip, double mu, double lambda,
+                             int variant, int coloring, double relaxation, int reverse,
+                             int symmetrized, double richardson_damping, int n_steps)
+{
+  if(dim < 2) return NULL;
+  or_ctx *c = or_create(dim, p, n, ip, variant, coloring, relaxation, reverse, symmetrized,
+                        richardson_damping, n_steps);
+  if(!c) return NULL;
+  c->ncomp = dim;
+  c->N = c->ncell * c->ndc * dim;
+  c->mu = mu;
+  c->lambda = lambda;
+  return c;
+}
+
+int or_n_components(const or_ctx *c) { return c->ncomp; }
 
 void or_destroy(or_ctx *c)
 {
@@ -695,8 +739,10 @@ int or_setup(or_ctx *c)
   c->vtx = (int *)malloc(sizeof(int) * P * dim);
   c->h = (double *)malloc(sizeof(double) * P * dim * 2);
   c->bdry = (unsigned char *)malloc(P * dim * 2);
-  c->S = (double *)malloc(sizeof(double) * P * dim * k * k);
-  c->lam = (double *)malloc(sizeof(double) * P * dim * k);
+  const int nb = c->ncomp;
+  if(nb > 1 && c->vertices) return -2; /* elasticity: Cartesian meshes only */
+  c->S = (double *)malloc(sizeof(double) * P * nb * dim * k * k);
+  c->lam = (double *)malloc(sizeof(double) * P * nb * dim * k);
 
   /* order: color, then interior before boundary, then vertex lexicographic */
   int pos = 0;
@@ -727,14 +773,24 @@ int or_setup(or_ctx *c)
     patch_surrogate_h(c, v, hh);
     for(int d = 0; d < dim; ++d)
     {
-      double M[OR_MAXK * OR_MAXK], L[OR_MAXK * OR_MAXK];
       const int bl = v[d] - 1 == 0, br = v[d] + 1 == n;
       c->h[(j * dim + d) * 2 + 0] = hh[2 * d];
       c->h[(j * dim + d) * 2 + 1] = hh[2 * d + 1];
       c->bdry[(j * dim + d) * 2 + 0] = (unsigned char)bl;
       c->bdry[(j * dim + d) * 2 + 1] = (unsigned char)br;
-      or_patch_matrices_1d(c->p, hh + 2 * d, bl, br, c->ip, M, L);
-      status |= or_gen_eig(k, L, M, c->S + (long)(j * dim + d) * k * k, c->lam + (long)(j * dim + d) * k);
+      for(int b = 0; b < nb; ++b)
+      {
+        double M[OR_MAXK * OR_MAXK], L[OR_MAXK * OR_MAXK];
+        double a_cons = 1.0, a_pen = 1.0;
+        if(nb > 1) /* block b of BlockMatrixDiagonal, linelasticity_integrator.h:636-645 */
+        {
+          a_cons = 2.0 * c->mu * (b == d ? 1.0 : 0.5) + (b == d ? c->lambda : 0.0);
+          a_pen = 2.0 * c->mu + (b == d ? c->lambda : 0.0);
+        }
+        or_patch_matrices_1d_coef(c->p, hh + 2 * d, bl, br, c->ip, a_cons, a_pen, M, L);
+        const long off = (long)(j * nb + b) * dim + d;
+        status |= or_gen_eig(k, L, M, c->S + off * k * k, c->lam + off * k);
+      }
     }
   }
   build_operator_1d(c);
@@ -759,8 +815,8 @@ void or_patch_vertices(const or_ctx *c, int *out) { memcpy(out, c->vtx, sizeof(i
 void or_patch_h(const or_ctx *c, double *out) { memcpy(out, c->h, sizeof(double) * c->P * c->dim * 2); }
 void or_patch_factors(const or_ctx *c, double *S, double *lam)
 {
-  if(S) memcpy(S, c->S, sizeof(double) * c->P * c->dim * c->k * c->k);
-  if(lam) memcpy(lam, c->lam, sizeof(double) * c->P * c->dim * c->k);
+  if(S) memcpy(S, c->S, sizeof(double) * c->P * c->ncomp * c->dim * c->k * c->k);
+  if(lam) memcpy(lam, c->lam, sizeof(double) * c->P * c->ncomp * c->dim * c->k);
 }
 
 /* Global DoF indices of one patch in patch-lexicographic order (A3):
@@ -786,11 +842,16 @@ static void patch_dofs(const or_ctx *c, const int *v, uint32_t *idx)
     }
     idx[i] = (uint32_t)(cell * c->ndc + loc);
   }
+  /* further components: the same cells in the next block (PatchTransferBlock
+   * juxtaposes the blocks' index sets, patch_transfer.h:340-354) */
+  const long Ns = c->ncell * c->ndc;
+  for(int b = 1; b < c->ncomp; ++b)
+    for(int i = 0; i < c->ndp; ++i) idx[b * c->ndp + i] = (uint32_t)(idx[i] + b * Ns);
 }
 
 void or_patch_dof_indices(const or_ctx *c, uint32_t *out)
 {
-  for(int j = 0; j < c->P; ++j) patch_dofs(c, c->vtx + j * c->dim, out + (long)j * c->ndp);
+  for(int j = 0; j < c->P; ++j) patch_dofs(c, c->vtx + j * c->dim, out + (long)j * c->ncomp * c->ndp);
 }
 
 /* get_color_sequence, schwarz_preconditioner.h:531-567 */
@@ -871,18 +932,27 @@ static void operator_cartesian(const or_ctx *c, double *dst, const double *src)
 
 static void operator_distorted(const or_ctx *c, double *dst, const double *src);
 
+static void operator_elasticity(const or_ctx *c, double *dst, const double *src);
+
 void or_operator_vmult(const or_ctx *c, double *dst, const double *src)
 {
-  if(c->vertices) operator_distorted(c, dst, src);
+  if(c->ncomp > 1) operator_elasticity(c, dst, src);
+  else if(c->vertices) operator_distorted(c, dst, src);
   else operator_cartesian(c, dst, src);
 }
 
 /* ---------------- Schwarz preconditioner (A1, A2, A12) ------------------- */
 
+/* A_j^{-1} r: one fast-diagonalization inverse per component block
+ * (BlockMatrixDiagonal::apply_inverse, block_matrix.h:1087-1130). */
 void or_local_apply(const or_ctx *c, int patch, const double *r, double *u)
 {
-  const long off = (long)patch * c->dim;
-  or_fd_apply(c->dim, c->k, c->S + off * c->k * c->k, c->lam + off * c->k, r, u);
+  for(int b = 0; b < c->ncomp; ++b)
+  {
+    const long off = (long)(patch * c->ncomp + b) * c->dim;
+    or_fd_apply(c->dim, c->k, c->S + off * c->k * c->k, c->lam + off * c->k, r + b * c->ndp,
+                u + b * c->ndp);
+  }
 }
 
 /* apply_local_solvers, schwarz_preconditioner.templates.h:585-662: gather,
@@ -890,7 +960,7 @@ void or_local_apply(const or_ctx *c, int patch, const double *r, double *u)
 static void apply_local_solvers(const or_ctx *c, double *solution, const double *residual, int color)
 {
   const int b = c->color_ptr[color], e = c->color_ptr[color + 1];
-  const int ndp = c->ndp;
+  const int ndp = c->ndp * c->ncomp;
   const int parallel = c->n_threads > 1 && c->coloring != OR_COLOR_NONE;
 #pragma omp parallel num_threads(parallel ? c->n_threads : 1)
   {
@@ -959,6 +1029,209 @@ void or_smoother_vmult(or_ctx *c, double *dst, const double *src)
 {
   memset(dst, 0, sizeof(double) * c->N);
   or_smoother_step(c, dst, src);
+}
+
+/* ========================================================================= */
+/* Linear elasticity SIPG operator (Cartesian)                               */
+/* ========================================================================= */
+
+/* out = (A_{dim-1} x .. x A_0)^T in (transposed 1D factors). */
+static void tensor_apply_t(int dim, int m, const double *const *mats, const double *in, double *out)
+{
+  double a[OR_MAXNP * OR_MAXNP * OR_MAXNP], b[OR_MAXNP * OR_MAXNP * OR_MAXNP];
+  const int n = ipow(m, dim);
+  memcpy(a, in, sizeof(double) * n);
+  for(int e = 0; e < dim; ++e)
+  {
+    mode_product(dim, m, e, mats[e], 1, a, b);
+    memcpy(a, b, sizeof(double) * n);
+  }
+  memcpy(out, a, sizeof(double) * n);
+}
+
+/*
+ * y = A x for the vector-valued SIPG discretization of
+ *   -div(2 mu eps(u) + lambda div(u) I) = f,  u = 0 on the boundary,
+ * assembled in the reference by deal.II's MeshWorker integrators
+ * (apps/linelasticity_integrator.h:31-124):
+ *   cell      2 mu eps(u):eps(v) + lambda div u div v
+ *             (LocalIntegrators::Elasticity::cell_matrix, GradDiv::cell_matrix)
+ *   interior  2 mu (sigma [u].[v] - {eps(u) n}.[v] - {eps(v) n}.[u])
+ *             + lambda (sigma [u.n][v.n] - {div u}[v.n] - {div v}[u.n])   (ip_matrix)
+ *   boundary  the same with a mirrored exterior state u_out = -u_in,
+ *             grad u_out = grad u_in, i.e. penalty 2 sigma     (nitsche_matrix)
+ * sigma = ip * gamma/h (Laplace::compute_penalty of equal cells), gamma = p(p+1).
+ * Written per cell and quadrature point with the full gradient tensor; every
+ * cell evaluates the traces of its face neighbours itself.
+ */
+static void operator_elasticity(const or_ctx *c, double *dst, const double *src)
+{
+  const int dim = c->dim, np = c->np, n = c->n, ndc = c->ndc, p = c->p;
+  const long Ns = c->ncell * ndc;
+  const double h = 1.0 / n, mu = c->mu, lam = c->lambda;
+  double nodes[OR_MAXNP], qx[OR_MAXNP], qw[OR_MAXNP];
+  double val[OR_MAXNP * OR_MAXNP], grad[OR_MAXNP * OR_MAXNP], fv[OR_MAXNP * 2], fg[OR_MAXNP * 2];
+  or_shape_info(p, nodes, qx, qw, val, grad, fv, fg);
+  /* square np x np factors, row = quadrature point, column = node */
+  double V[OR_MAXNP * OR_MAXNP], G[OR_MAXNP * OR_MAXNP];
+  double FV[2][OR_MAXNP * OR_MAXNP], FG[2][OR_MAXNP * OR_MAXNP];
+  for(int q = 0; q < np; ++q)
+    for(int i = 0; i < np; ++i)
+    {
+      V[q * np + i] = val[i * np + q];
+      G[q * np + i] = grad[i * np + q] / h;
+      for(int f = 0; f < 2; ++f)
+      {
+        FV[f][q * np + i] = fv[i * 2 + f]; /* every row: the trace at x = f */
+        FG[f][q * np + i] = fg[i * 2 + f] / h;
+      }
+    }
+  const double sig = c->ip * 0.5 * or_penalty(p, h, h);
+
+#pragma omp parallel for num_threads(c->n_threads) schedule(static)
+  for(long cell = 0; cell < c->ncell; ++cell)
+  {
+    enum { MX = OR_MAXNP * OR_MAXNP * OR_MAXNP };
+    double u[3][MX], acc[3][MX], gu[3][3][MX], gw[3][3][MX], tmp[MX], t2[MX];
+    int cc[3];
+    long r = cell;
+    for(int d = 0; d < dim; ++d) { cc[d] = (int)(r % n); r /= n; }
+    for(int b = 0; b < dim; ++b)
+      for(int i = 0; i < ndc; ++i)
+      {
+        u[b][i] = src[b * Ns + cell * ndc + i];
+        acc[b][i] = 0.0;
+      }
+
+    /* cell term: stress at the Gauss points times JxW, tested with grad v */
+    for(int b = 0; b < dim; ++b)
+      for(int j = 0; j < dim; ++j)
+      {
+        const double *m[3];
+        for(int e = 0; e < dim; ++e) m[e] = e == j ? G : V;
+        tensor_apply(dim, np, m, u[b], gu[b][j], tmp);
+      }
+    for(int q = 0; q < ndc; ++q)
+    {
+      double w = 1.0;
+      int t = q;
+      for(int e = 0; e < dim; ++e) { w *= qw[t % np] * h; t /= np; }
+      double div = 0.0;
+      for(int b = 0; b < dim; ++b) div += gu[b][b][q];
+      for(int b = 0; b < dim; ++b)
+        for(int j = 0; j < dim; ++j)
+          gw[b][j][q] = w * (mu * (gu[b][j][q] + gu[j][b][q]) + (b == j ? lam * div : 0.0));
+    }
+    for(int b = 0; b < dim; ++b)
+      for(int j = 0; j < dim; ++j)
+      {
+        const double *m[3];
+        for(int e = 0; e < dim; ++e) m[e] = e == j ? G : V;
+        tensor_apply_t(dim, np, m, gw[b][j], tmp);
+        for(int i = 0; i < ndc; ++i) acc[b][i] += tmp[i];
+      }
+
+    /* faces */
+    long stride = 1;
+    for(int d = 0; d < dim; ++d)
+    {
+      for(int f = 0; f < 2; ++f)
+      {
+        const double nrm = f ? 1.0 : -1.0;
+        const int interior = f ? cc[d] < n - 1 : cc[d] > 0;
+        const long nb = f ? cell + stride : cell - stride;
+        /* traces of this cell (face f) and of the exterior state */
+        double U[3][MX], GU[3][3][MX], Uo[3][MX], GUo[3][3][MX];
+        for(int b = 0; b < dim; ++b)
+        {
+          const double *m[3];
+          for(int e = 0; e < dim; ++e) m[e] = e == d ? FV[f] : V;
+          tensor_apply(dim, np, m, u[b], U[b], tmp);
+          for(int j = 0; j < dim; ++j)
+          {
+            for(int e = 0; e < dim; ++e) m[e] = e == d ? (j == d ? FG[f] : FV[f]) : (e == j ? G : V);
+            tensor_apply(dim, np, m, u[b], GU[b][j], tmp);
+          }
+          if(interior)
+          {
+            double un[MX];
+            for(int i = 0; i < ndc; ++i) un[i] = src[b * Ns + nb * ndc + i];
+            for(int e = 0; e < dim; ++e) m[e] = e == d ? FV[1 - f] : V;
+            tensor_apply(dim, np, m, un, Uo[b], tmp);
+            for(int j = 0; j < dim; ++j)
+            {
+              for(int e = 0; e < dim; ++e)
+                m[e] = e == d ? (j == d ? FG[1 - f] : FV[1 - f]) : (e == j ? G : V);
+              tensor_apply(dim, np, m, un, GUo[b][j], tmp);
+            }
+          }
+          else
+            for(int i = 0; i < ndc; ++i)
+            {
+              Uo[b][i] = -U[b][i];
+              for(int j = 0; j < dim; ++j) GUo[b][j][i] = GU[b][j][i];
+            }
+        }
+        /* coefficients of v_b (A) and of d_j v_b (B) at the face points */
+        double A[3][MX], B[3][3][MX];
+        for(int b = 0; b < dim; ++b)
+          for(int i = 0; i < ndc; ++i)
+          {
+            A[b][i] = 0.0;
+            for(int j = 0; j < dim; ++j) B[b][j][i] = 0.0;
+          }
+        for(int q = 0; q < ndc; ++q)
+        {
+          int t = q, on_slice = 1;
+          double w = 1.0;
+          for(int e = 0; e < dim; ++e)
+          {
+            const int qe = t % np;
+            t /= np;
+            if(e == d) on_slice = qe == 0;
+            else w *= qw[qe] * h;
+          }
+          if(!on_slice) continue;
+          double J[3], divi = 0.0, divo = 0.0;
+          for(int b = 0; b < dim; ++b)
+          {
+            J[b] = U[b][q] - Uo[b][q];
+            divi += GU[b][b][q];
+            divo += GUo[b][b][q];
+          }
+          for(int b = 0; b < dim; ++b)
+          {
+            /* {2 mu eps(u) n + lambda div(u) n}_b */
+            const double fi = mu * (GU[b][d][q] + GU[d][b][q]) + (b == d ? lam * divi : 0.0);
+            const double fo = mu * (GUo[b][d][q] + GUo[d][b][q]) + (b == d ? lam * divo : 0.0);
+            const double F = 0.5 * (fi + fo) * nrm;
+            A[b][q] = w * (-F + 2.0 * mu * sig * J[b] + (b == d ? lam * sig * J[d] : 0.0));
+            /* -1/2 (2 mu eps(v) n . [u] + lambda div v [u].n) */
+            B[b][d][q] += w * (-0.5 * mu * nrm * J[b]);
+            if(b == d)
+              for(int j = 0; j < dim; ++j) B[b][j][q] += w * (-0.5 * mu * nrm * J[j]);
+            B[b][b][q] += w * (-0.5 * lam * nrm * J[d]);
+          }
+        }
+        for(int b = 0; b < dim; ++b)
+        {
+          const double *m[3];
+          for(int e = 0; e < dim; ++e) m[e] = e == d ? FV[f] : V;
+          tensor_apply_t(dim, np, m, A[b], t2);
+          for(int i = 0; i < ndc; ++i) acc[b][i] += t2[i];
+          for(int j = 0; j < dim; ++j)
+          {
+            for(int e = 0; e < dim; ++e) m[e] = e == d ? (j == d ? FG[f] : FV[f]) : (e == j ? G : V);
+            tensor_apply_t(dim, np, m, B[b][j], t2);
+            for(int i = 0; i < ndc; ++i) acc[b][i] += t2[i];
+          }
+        }
+      }
+      stride *= n;
+    }
+    for(int b = 0; b < dim; ++b)
+      for(int i = 0; i < ndc; ++i) dst[b * Ns + cell * ndc + i] = acc[b][i];
+  }
 }
 
 /* ========================================================================= */
@@ -1318,7 +1591,7 @@ void or_restrict(int dim, int p, int n_coarse, const double *xf, double *xc)
 
 struct or_gmg
 {
-  int dim, p, n_levels;
+  int dim, p, n_levels, ncomp;
   double coarse_tol;
   or_ctx **pre, **post;
   double **r, **x, **b, **t;
@@ -1348,14 +1621,13 @@ static double *coarsen_vertices(int dim, int nf, const double *vf)
   return vc;
 }
 
-or_gmg *or_gmg_create_v(int dim, int p, int n_coarse, int n_levels, double ip,
-                        int variant, int coloring, double relaxation,
-                        int n_steps, double coarse_tol, const double *vertices)
+static or_gmg *gmg_alloc(int dim, int p, int n_levels, int ncomp, double coarse_tol)
 {
   or_gmg *g = (or_gmg *)calloc(1, sizeof(or_gmg));
   g->dim = dim;
   g->p = p;
   g->n_levels = n_levels;
+  g->ncomp = ncomp;
   g->coarse_tol = coarse_tol;
   g->pre = (or_ctx **)calloc(n_levels, sizeof(or_ctx *));
   g->post = (or_ctx **)calloc(n_levels, sizeof(or_ctx *));
@@ -1363,6 +1635,25 @@ or_gmg *or_gmg_create_v(int dim, int p, int n_coarse, int n_levels, double ip,
   g->x = (double **)calloc(n_levels, sizeof(double *));
   g->b = (double **)calloc(n_levels, sizeof(double *));
   g->t = (double **)calloc(n_levels, sizeof(double *));
+  return g;
+}
+
+static void gmg_level_setup(or_gmg *g, int l)
+{
+  or_setup(g->pre[l]);
+  or_setup(g->post[l]);
+  const long N = g->pre[l]->N;
+  g->r[l] = (double *)calloc(N, sizeof(double));
+  g->x[l] = (double *)calloc(N, sizeof(double));
+  g->b[l] = (double *)calloc(N, sizeof(double));
+  g->t[l] = (double *)calloc(N, sizeof(double));
+}
+
+or_gmg *or_gmg_create_v(int dim, int p, int n_coarse, int n_levels, double ip,
+                        int variant, int coloring, double relaxation,
+                        int n_steps, double coarse_tol, const double *vertices)
+{
+  or_gmg *g = gmg_alloc(dim, p, n_levels, 1, coarse_tol);
   double **verts = (double **)calloc(n_levels, sizeof(double *));
   if(vertices)
   {
@@ -1385,15 +1676,28 @@ or_gmg *or_gmg_create_v(int dim, int p, int n_coarse, int n_levels, double ip,
       or_set_vertices(g->post[l], verts[l]);
       free(verts[l]);
     }
-    or_setup(g->pre[l]);
-    or_setup(g->post[l]);
-    const long N = g->pre[l]->N;
-    g->r[l] = (double *)calloc(N, sizeof(double));
-    g->x[l] = (double *)calloc(N, sizeof(double));
-    g->b[l] = (double *)calloc(N, sizeof(double));
-    g->t[l] = (double *)calloc(N, sizeof(double));
+    gmg_level_setup(g, l);
   }
   free(verts);
+  return g;
+}
+
+/* GMG for linear elasticity: the same hierarchy, level smoothers with the
+ * block-diagonal local solvers, transfers applied per component block. */
+or_gmg *or_gmg_create_elasticity(int dim, int p, int n_coarse, int n_levels, double ip,
+                                 double mu, double lambda, int variant, int coloring,
+                                 double relaxation, int n_steps, double coarse_tol)
+{
+  or_gmg *g = gmg_alloc(dim, p, n_levels, dim, coarse_tol);
+  for(int l = 0; l < n_levels; ++l)
+  {
+    const int n = n_coarse << l;
+    g->pre[l] = or_create_elasticity(dim, p, n, ip, mu, lambda, variant, coloring, relaxation, 0,
+                                     0, 1.0, n_steps);
+    g->post[l] = or_create_elasticity(dim, p, n, ip, mu, lambda, variant, coloring, relaxation, 1,
+                                      0, 1.0, n_steps);
+    gmg_level_setup(g, l);
+  }
   return g;
 }
 
@@ -1462,9 +1766,12 @@ static void vcycle_level(or_gmg *g, int l, double *x, const double *b)
   or_operator_vmult(c, g->r[l], x);
   for(long i = 0; i < N; ++i) g->r[l][i] = b[i] - g->r[l][i];
   const int nc = g->pre[l - 1]->n;
-  or_restrict(g->dim, g->p, nc, g->r[l], g->b[l - 1]);
+  const long Nsf = N / g->ncomp, Nsc = g->pre[l - 1]->N / g->ncomp;
+  for(int b = 0; b < g->ncomp; ++b)
+    or_restrict(g->dim, g->p, nc, g->r[l] + b * Nsf, g->b[l - 1] + b * Nsc);
   vcycle_level(g, l - 1, g->x[l - 1], g->b[l - 1]);
-  or_prolongate_add(g->dim, g->p, nc, g->x[l - 1], x);
+  for(int b = 0; b < g->ncomp; ++b)
+    or_prolongate_add(g->dim, g->p, nc, g->x[l - 1] + b * Nsc, x + b * Nsf);
   or_smoother_step(g->post[l], x, b);
 }
 

@@ -38,6 +38,8 @@ void or_shape_info(int p, double *nodes, double *qx, double *qw, double *val,
 double or_penalty(int p, double h_left, double h_right);
 void or_patch_matrices_1d(int p, const double *h, int bdry_left, int bdry_right,
                           double ip, double *M, double *L);
+void or_patch_matrices_1d_coef(int p, const double *h, int bdry_left, int bdry_right,
+                               double ip, double a_cons, double a_pen, double *M, double *L);
 int or_gen_eig(int k, const double *A, const double *B, double *S, double *lam);
 void or_fd_apply(int dim, int k, const double *S, const double *lam,
                  const double *in, double *out);
@@ -48,6 +50,11 @@ typedef struct or_ctx or_ctx;
 or_ctx *or_create(int dim, int p, int n, double ip, int variant, int coloring,
                   double relaxation, int reverse, int symmetrized,
                   double richardson_damping, int n_steps);
+/* linear elasticity: dim components, component-blocked vectors, Cartesian */
+or_ctx *or_create_elasticity(int dim, int p, int n, double ip, double mu, double lambda,
+                             int variant, int coloring, double relaxation, int reverse,
+                             int symmetrized, double richardson_damping, int n_steps);
+int or_n_components(const or_ctx *c);
 void or_destroy(or_ctx *c);
 int or_set_vertices(or_ctx *c, const double *coords);
 int or_setup(or_ctx *c);
@@ -82,6 +89,9 @@ or_gmg *or_gmg_create(int dim, int p, int n_coarse, int n_levels, double ip,
 or_gmg *or_gmg_create_v(int dim, int p, int n_coarse, int n_levels, double ip,
                         int variant, int coloring, double relaxation,
                         int n_steps, double coarse_tol, const double *vertices);
+or_gmg *or_gmg_create_elasticity(int dim, int p, int n_coarse, int n_levels, double ip,
+                                 double mu, double lambda, int variant, int coloring,
+                                 double relaxation, int n_steps, double coarse_tol);
 void or_gmg_destroy(or_gmg *g);
 or_ctx *or_gmg_level(or_gmg *g, int level);
 void or_gmg_vcycle(or_gmg *g, double *dst, const double *src);

@@ -22,6 +22,7 @@ OK, E_INVALID_ARG, E_NONFINITE, E_SINGULAR, E_HIP, E_NOT_CONVERGED, E_UNSUPPORTE
 ADDITIVE, MULTIPLICATIVE = 0, 1
 COLOR_NONE, COLOR_TILED, COLOR_REDBLACK = 0, 1, 2
 F64, F32, BF16 = 0, 1, 2
+POISSON, ELASTICITY = 0, 1
 
 #: TPSS::lookup_damping_factor (TPSS.h:120-163) for vertex patches
 DEFAULT_DAMPING = {(ADDITIVE, 2): 0.25, (ADDITIVE, 3): 0.15,
@@ -37,7 +38,8 @@ class TPSSError(RuntimeError):
 class LevelDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("degree", ctypes.c_int), ("n_cells_1d", ctypes.c_int),
                 ("ip_factor", ctypes.c_double), ("vertices", ctypes.POINTER(ctypes.c_double)),
-                ("vector_precision", ctypes.c_int)]
+                ("vector_precision", ctypes.c_int), ("equation", ctypes.c_int),
+                ("mu", ctypes.c_double), ("lambda_", ctypes.c_double)]
 
 
 class SmootherDesc(ctypes.Structure):
@@ -56,7 +58,8 @@ class GMGDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("degree", ctypes.c_int), ("n_cells_coarse", ctypes.c_int),
                 ("n_levels", ctypes.c_int), ("ip_factor", ctypes.c_double),
                 ("vertices", ctypes.POINTER(ctypes.c_double)), ("mg_precision", ctypes.c_int),
-                ("smoother", SmootherDesc), ("coarse_tol", ctypes.c_double)]
+                ("smoother", SmootherDesc), ("coarse_tol", ctypes.c_double),
+                ("equation", ctypes.c_int), ("mu", ctypes.c_double), ("lambda_", ctypes.c_double)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -269,11 +272,17 @@ def _smoother_desc(variant, coloring, relaxation, reverse, symmetrized, damping_
 
 
 class Level:
-    """Mesh + DG Q_p space + SIPG operator of one level (device)."""
+    """Mesh + DG Q_p space + SIPG operator of one level (device).
 
-    def __init__(self, dim, degree, n_cells_1d, ip_factor=1.0, vertices=None, precision=F64):
+    elasticity=(mu, lambda) selects the linear elasticity operator with dim
+    component blocks instead of the Poisson operator."""
+
+    def __init__(self, dim, degree, n_cells_1d, ip_factor=1.0, vertices=None, precision=F64,
+                 elasticity=None):
         self._varr, vp = _vertices_ptr(vertices)
-        desc = LevelDesc(dim, degree, n_cells_1d, float(ip_factor), vp, precision)
+        eq, mu, lam = (POISSON, 0.0, 0.0) if elasticity is None else (ELASTICITY,) + tuple(
+            float(v) for v in elasticity)
+        desc = LevelDesc(dim, degree, n_cells_1d, float(ip_factor), vp, precision, eq, mu, lam)
         h = ctypes.c_void_p()
         _check(lib.tpss_level_create(ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h
@@ -445,7 +454,8 @@ class GMGSolver:
 
     def __init__(self, dim, degree, n_cells_coarse, n_levels, ip_factor=1.0, vertices=None,
                  mg_precision=F64, variant=MULTIPLICATIVE, coloring=None, relaxation=None,
-                 damping_factor=1.0, n_steps=1, local_precision=None, coarse_tol=1e-12):
+                 damping_factor=1.0, n_steps=1, local_precision=None, coarse_tol=1e-12,
+                 elasticity=None):
         if coloring is None:
             coloring = COLOR_NONE if variant == ADDITIVE else COLOR_REDBLACK
         if relaxation is None:
@@ -455,14 +465,17 @@ class GMGSolver:
         self._varr, vp = _vertices_ptr(vertices)
         sd = _smoother_desc(variant, coloring, relaxation, False, False, damping_factor, n_steps,
                             local_precision)
+        eq, mu, lam = (POISSON, 0.0, 0.0) if elasticity is None else (ELASTICITY,) + tuple(
+            float(v) for v in elasticity)
         self.desc = GMGDesc(dim, degree, n_cells_coarse, n_levels, float(ip_factor), vp,
-                            mg_precision, sd, float(coarse_tol))
+                            mg_precision, sd, float(coarse_tol), eq, mu, lam)
         h = ctypes.c_void_p()
         _check(lib.tpss_gmg_create(ctypes.byref(self.desc), ctypes.byref(h)))
         self.handle = h
         self.dim, self.degree, self.n_levels = dim, degree, n_levels
         self.n_fine = n_cells_coarse << (n_levels - 1)
-        self.n_dofs = self.n_fine ** dim * (degree + 1) ** dim
+        self.n_components = 1 if elasticity is None else dim
+        self.n_dofs = self.n_components * self.n_fine ** dim * (degree + 1) ** dim
         self.mg_precision = mg_precision
 
     def level(self, l):

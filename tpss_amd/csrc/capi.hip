@@ -16,6 +16,7 @@
 #include "setup_kernels.hpp"
 #include "ops.hpp"
 #include "kernels_general.hpp"
+#include "kernels_elasticity.hpp"
 
 namespace tpss
 {
@@ -46,10 +47,58 @@ constexpr int cells_per_block()
 }
 
 // y = A x (or y = b - A x), optionally only on a list of cells; vectors of type VT
+template<int NP>
+ElastTab<NP> make_elast_tab(const tpss_level *lv)
+{
+  ElastTab<NP> tab;
+  const Basis1D bs = make_basis(NP - 1);
+  const double h = lv->h;
+  for(int q = 0; q < NP; ++q)
+  {
+    tab.w[q] = h * bs.qw[q];
+    for(int i = 0; i < NP; ++i)
+    {
+      tab.V[q * NP + i] = bs.val[i * NP + q];
+      tab.G[q * NP + i] = bs.grad[i * NP + q] / h;
+    }
+  }
+  for(int f = 0; f < 2; ++f)
+    for(int i = 0; i < NP; ++i)
+    {
+      tab.f[f][i] = bs.fval[i * 2 + f];
+      tab.fd[f][i] = bs.fgrad[i * 2 + f] / h;
+    }
+  const int p = NP - 1;
+  const double gamma = p == 0 ? 1.0 : double(p) * (p + 1);
+  tab.mu = lv->mu;
+  tab.lambda = lv->lambda;
+  tab.sigma = lv->ip * gamma / h;  // Laplace::compute_penalty of two cells of size h
+  return tab;
+}
+
 template<typename VT>
 void operator_apply(const tpss_level *lv, VT *y, const VT *x, const VT *b, const uint32_t *cells,
                     int64_t n_list, hipStream_t s)
 {
+  if(lv->ncomp > 1)
+  {
+    dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
+      constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
+      constexpr int NDC = cpow(NP, DIM);
+      const auto tab = make_elast_tab<NP>(lv);
+      const int64_t count = cells ? n_list : lv->ncell;
+      if(count == 0) return;
+      const unsigned threads = NDC <= 64 ? 64u : 128u;
+      if(b)
+        hipLaunchKernelGGL((k_elasticity<DIM, NP, VT, true>), dim3(unsigned(count)), dim3(threads),
+                           0, s, x, y, b, tab, lv->n, lv->Ns, cells);
+      else
+        hipLaunchKernelGGL((k_elasticity<DIM, NP, VT, false>), dim3(unsigned(count)),
+                           dim3(threads), 0, s, x, y, b, tab, lv->n, lv->Ns, cells);
+      TPSS_HIP(hipGetLastError());
+    });
+    return;
+  }
   if(lv->distorted)
   {
     dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
@@ -200,6 +249,14 @@ int tpss_level_create(const tpss_level_desc *desc, tpss_level **out)
     TPSS_REQUIRE(desc->ip_factor > 0, "ip_factor must be positive");
     TPSS_REQUIRE(desc->vector_precision == TPSS_F64 || desc->vector_precision == TPSS_F32,
                  "vector precision must be F64 or F32");
+    TPSS_REQUIRE(desc->equation == TPSS_POISSON || desc->equation == TPSS_ELASTICITY,
+                 "invalid equation");
+    const bool elasticity = desc->equation == TPSS_ELASTICITY;
+    if(elasticity)
+    {
+      TPSS_REQUIRE(!desc->vertices, "linear elasticity: Cartesian meshes only (vertices = NULL)");
+      TPSS_REQUIRE(desc->mu > 0.0 && desc->lambda >= 0.0, "Lame coefficients: mu > 0, lambda >= 0");
+    }
     auto lv = std::make_unique<tpss_level>();
     lv->dim = desc->dim;
     lv->p = desc->degree;
@@ -207,7 +264,11 @@ int tpss_level_create(const tpss_level_desc *desc, tpss_level **out)
     lv->n = desc->n_cells_1d;
     lv->ndc = ipow(lv->np, lv->dim);
     lv->ncell = ipow(lv->n, lv->dim);
-    lv->N = lv->ncell * lv->ndc;
+    lv->Ns = lv->ncell * lv->ndc;
+    lv->ncomp = elasticity ? lv->dim : 1;
+    lv->N = lv->Ns * lv->ncomp;
+    lv->mu = elasticity ? desc->mu : 0.0;
+    lv->lambda = elasticity ? desc->lambda : 0.0;
     lv->ip = desc->ip_factor;
     lv->h = 1.0 / lv->n;
     lv->vec_prec = desc->vector_precision;
@@ -331,7 +392,7 @@ int tpss_transfer_prolongate_add(tpss_level *coarse, tpss_level *fine, void *dst
   return guarded([&] {
     TPSS_REQUIRE(coarse && fine && dst_fine && src_coarse, "null argument");
     TPSS_REQUIRE(fine->n == 2 * coarse->n && fine->dim == coarse->dim && fine->p == coarse->p &&
-                   fine->vec_prec == coarse->vec_prec,
+                   fine->vec_prec == coarse->vec_prec && fine->ncomp == coarse->ncomp,
                  "levels are not a refinement pair");
     hipStream_t s = as_stream(stream);
     dispatch_dim_np(fine->dim, fine->np, [&](auto D, auto NPc) {
@@ -341,14 +402,20 @@ int tpss_transfer_prolongate_add(tpss_level *coarse, tpss_level *fine, void *dst
       prolongation_1d(NP - 1, P);
       for(int c = 0; c < 2; ++c)
         for(int e = 0; e < NP * NP; ++e) tab.P[c][e] = P[c * NP * NP + e];
-      const unsigned g = blocks_for(fine->N, 256);
-      if(fine->vec_prec == TPSS_F64)
-        hipLaunchKernelGGL((k_prolongate_add<DIM, NP, double>), dim3(g), dim3(256), 0, s,
-                           (double *)dst_fine, (const double *)src_coarse, tab, coarse->n, fine->N);
-      else
-        hipLaunchKernelGGL((k_prolongate_add<DIM, NP, float>), dim3(g), dim3(256), 0, s,
-                           (float *)dst_fine, (const float *)src_coarse, tab, coarse->n, fine->N);
-      TPSS_HIP(hipGetLastError());
+      // one scalar DG transfer per component block (MGTransferMatrixFree per block)
+      const unsigned g = blocks_for(fine->Ns, 256);
+      for(int b = 0; b < fine->ncomp; ++b)
+      {
+        if(fine->vec_prec == TPSS_F64)
+          hipLaunchKernelGGL((k_prolongate_add<DIM, NP, double>), dim3(g), dim3(256), 0, s,
+                             (double *)dst_fine + b * fine->Ns,
+                             (const double *)src_coarse + b * coarse->Ns, tab, coarse->n, fine->Ns);
+        else
+          hipLaunchKernelGGL((k_prolongate_add<DIM, NP, float>), dim3(g), dim3(256), 0, s,
+                             (float *)dst_fine + b * fine->Ns,
+                             (const float *)src_coarse + b * coarse->Ns, tab, coarse->n, fine->Ns);
+        TPSS_HIP(hipGetLastError());
+      }
     });
   });
 }
@@ -359,7 +426,7 @@ int tpss_transfer_restrict(tpss_level *coarse, tpss_level *fine, void *dst_coars
   return guarded([&] {
     TPSS_REQUIRE(coarse && fine && dst_coarse && src_fine, "null argument");
     TPSS_REQUIRE(fine->n == 2 * coarse->n && fine->dim == coarse->dim && fine->p == coarse->p &&
-                   fine->vec_prec == coarse->vec_prec,
+                   fine->vec_prec == coarse->vec_prec && fine->ncomp == coarse->ncomp,
                  "levels are not a refinement pair");
     hipStream_t s = as_stream(stream);
     dispatch_dim_np(fine->dim, fine->np, [&](auto D, auto NPc) {
@@ -369,14 +436,19 @@ int tpss_transfer_restrict(tpss_level *coarse, tpss_level *fine, void *dst_coars
       prolongation_1d(NP - 1, P);
       for(int c = 0; c < 2; ++c)
         for(int e = 0; e < NP * NP; ++e) tab.P[c][e] = P[c * NP * NP + e];
-      const unsigned g = blocks_for(coarse->N, 256);
-      if(fine->vec_prec == TPSS_F64)
-        hipLaunchKernelGGL((k_restrict<DIM, NP, double>), dim3(g), dim3(256), 0, s,
-                           (double *)dst_coarse, (const double *)src_fine, tab, coarse->n, coarse->N);
-      else
-        hipLaunchKernelGGL((k_restrict<DIM, NP, float>), dim3(g), dim3(256), 0, s,
-                           (float *)dst_coarse, (const float *)src_fine, tab, coarse->n, coarse->N);
-      TPSS_HIP(hipGetLastError());
+      const unsigned g = blocks_for(coarse->Ns, 256);
+      for(int b = 0; b < fine->ncomp; ++b)
+      {
+        if(fine->vec_prec == TPSS_F64)
+          hipLaunchKernelGGL((k_restrict<DIM, NP, double>), dim3(g), dim3(256), 0, s,
+                             (double *)dst_coarse + b * coarse->Ns,
+                             (const double *)src_fine + b * fine->Ns, tab, coarse->n, coarse->Ns);
+        else
+          hipLaunchKernelGGL((k_restrict<DIM, NP, float>), dim3(g), dim3(256), 0, s,
+                             (float *)dst_coarse + b * coarse->Ns,
+                             (const float *)src_fine + b * fine->Ns, tab, coarse->n, coarse->Ns);
+        TPSS_HIP(hipGetLastError());
+      }
     });
   });
 }

@@ -176,6 +176,9 @@ struct tpss_level
 {
   int dim = 0, p = 0, np = 0, n = 0, ndc = 0;
   int64_t ncell = 0, N = 0;
+  int ncomp = 1;          // 1: Poisson; dim: linear elasticity (component blocks)
+  int64_t Ns = 0;         // DoFs of one component block (ncell * ndc)
+  double mu = 0.0, lambda = 0.0;
   double ip = 1.0, h = 0.0;
   int vec_prec = TPSS_F64;
   bool distorted = false;

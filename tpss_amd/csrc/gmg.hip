@@ -68,6 +68,9 @@ tpss_level *make_level(const tpss_gmg_desc &d, int n, const std::vector<double> 
   ld.ip_factor = d.ip_factor;
   ld.vertices = verts ? verts->data() : nullptr;
   ld.vector_precision = prec;
+  ld.equation = d.equation;
+  ld.mu = d.mu;
+  ld.lambda = d.lambda;
   tpss_level *lv = nullptr;
   const int st = tpss_level_create(&ld, &lv);
   if(st != TPSS_OK) throw Error(st, std::string("level: ") + tpss_last_error());

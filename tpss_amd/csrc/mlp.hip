@@ -704,6 +704,7 @@ int tpss_smoother_patch_features(const tpss_smoother *sm, float *dev_out, void *
 {
   return guarded([&] {
     TPSS_REQUIRE(sm && dev_out, "null argument");
+    TPSS_REQUIRE(sm->level->ncomp == 1, "learned damping: Poisson levels only");
     patch_features(sm, dev_out, as_stream(stream));
   });
 }
@@ -713,6 +714,7 @@ int tpss_smoother_optimal_damping(tpss_smoother *sm, int n_samples, uint64_t see
 {
   return guarded([&] {
     TPSS_REQUIRE(sm && dev_out && n_samples >= 1, "invalid arguments");
+    TPSS_REQUIRE(sm->level->ncomp == 1, "learned damping: Poisson levels only");
     TPSS_REQUIRE(omega_max > omega_min, "empty damping range");
     optimal_damping(sm, n_samples, seed, omega_min, omega_max, dev_out, nullptr);
   });

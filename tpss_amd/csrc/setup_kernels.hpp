@@ -25,10 +25,15 @@ struct Shape1D
 // SIPG face operation with average factor 1 at the physical boundary and 1/2
 // otherwise (common_integrator.h:196-240, fd_evaluation.h:651-688), interface
 // cell 0 | cell 1 (common_integrator.h:242-289), DGQ 1D index c*(p+1)+i.
+// L = a_cons * (stiffness + consistency terms) + a_pen * (penalty terms): (1, 1)
+// for Poisson; for component c of linear elasticity in direction d
+// a_cons = 2 mu (1 if c == d else 1/2) + lambda [c == d], a_pen = 2 mu + lambda [c == d]
+// (CellStrain, NitscheStrain, CellGradDiv, NitscheGradDiv,
+// apps/linelasticity_integrator.h:753-1145, basic penalty variant).
 template<int NP>
 __device__ void assemble_patch_1d(const Shape1D<NP> &sh, const int p, const double h0,
                                   const double h1, const bool bl, const bool br, const double ip,
-                                  double *M, double *L)
+                                  const double a_cons, const double a_pen, double *M, double *L)
 {
   constexpr int K = 2 * NP;
   const double gamma = p == 0 ? 1.0 : double(p) * (p + 1);
@@ -51,7 +56,7 @@ __device__ void assemble_patch_1d(const Shape1D<NP> &sh, const int p, const doub
           l += (sh.grad[i * NP + q] / h) * (sh.grad[jj * NP + q] / h) * dx;
         }
         M[(c * NP + i) * K + c * NP + jj] += m;
-        L[(c * NP + i) * K + c * NP + jj] += l;
+        L[(c * NP + i) * K + c * NP + jj] += a_cons * l;
       }
     for(int f = 0; f < 2; ++f)
     {
@@ -67,7 +72,7 @@ __device__ void assemble_patch_1d(const Shape1D<NP> &sh, const int p, const doub
         {
           const double uj = sh.fval[jj * 2 + f], guj = sh.fgrad[jj * 2 + f] / h;
           L[(c * NP + i) * K + c * NP + jj] +=
-            -a * (vi * nrm * guj + gi * uj * nrm) + pen * vi * uj * nrm * nrm;
+            -a_cons * a * (vi * nrm * guj + gi * uj * nrm) + a_pen * pen * vi * uj * nrm * nrm;
         }
       }
     }
@@ -82,14 +87,17 @@ __device__ void assemble_patch_1d(const Shape1D<NP> &sh, const int p, const doub
       const double u0 = sh.fval[jj * 2 + 1], gu0 = sh.fgrad[jj * 2 + 1] / h0;
       const double u1 = sh.fval[jj * 2 + 0], gu1 = sh.fgrad[jj * 2 + 0] / h1;
       // normals: +1 on cell 0, -1 on cell 1
-      L[i * K + NP + jj] += -0.5 * (v0 * gu1 - gv0 * u1) - pen * v0 * u1;
-      L[(NP + i) * K + jj] += -0.5 * (-v1 * gu0 + gv1 * u0) - pen * v1 * u0;
+      L[i * K + NP + jj] += -a_cons * 0.5 * (v0 * gu1 - gv0 * u1) - a_pen * pen * v0 * u1;
+      L[(NP + i) * K + jj] += -a_cons * 0.5 * (-v1 * gu0 + gv1 * u0) - a_pen * pen * v1 * u0;
     }
   }
 }
 
 // L s = lambda M s: Cholesky M = C C^T, cyclic Jacobi on C^{-1} L C^{-T},
-// S = C^{-T} Q, ascending eigenvalues.  Returns false if M is not s.p.d.
+// S = C^{-T} Q, ascending eigenvalues.  Returns false if M is not s.p.d. or an
+// eigenvalue is not finite.  A 1D eigenvalue may be negative (an SIPG matrix of
+// a distorted patch direction can be indefinite); what must not vanish is the
+// sum over directions, checked per patch by k_check_eigenvalue_sums.
 template<int K>
 __device__ bool gen_eig(const double *L, const double *M, double *S, double *lam)
 {
@@ -192,7 +200,7 @@ __device__ bool gen_eig(const double *L, const double *M, double *S, double *lam
       for(int m = i + 1; m < K; ++m) t -= C[m * K + i] * S[m * K + c];
       S[i * K + c] = t / C[i * K + i];
     }
-    if(!(lam[c] > 0.0)) return false;
+    if(!isfinite(lam[c])) return false;
   }
   return true;
 }
@@ -200,20 +208,52 @@ __device__ bool gen_eig(const double *L, const double *M, double *S, double *lam
 // one thread per 1D problem (h0, h1, boundary bits)
 template<int NP>
 __global__ void __launch_bounds__(64)
-k_setup_fd(const double *__restrict__ h, const uint8_t *__restrict__ bdry, const Shape1D<NP> sh,
-           const int p, const double ip, const int nprob, double *__restrict__ S_out,
-           double *__restrict__ lam_out, int *__restrict__ status)
+k_setup_fd(const double *__restrict__ h, const uint8_t *__restrict__ bdry,
+           const double *__restrict__ coef, const Shape1D<NP> sh, const int p, const double ip,
+           const int nprob, double *__restrict__ S_out, double *__restrict__ lam_out,
+           int *__restrict__ status)
 {
   constexpr int K = 2 * NP;
   const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
   if(i >= nprob) return;
   double M[K * K], L[K * K], S[K * K], lam[K];
   assemble_patch_1d<NP>(sh, p, h[2 * i], h[2 * i + 1], bdry[2 * i] != 0, bdry[2 * i + 1] != 0,
-                        ip, M, L);
+                        ip, coef[2 * i], coef[2 * i + 1], M, L);
   const bool ok = gen_eig<K>(L, M, S, lam);
   if(!ok) status[0] = 1;
   for(int e = 0; e < K * K; ++e) S_out[size_t(i) * K * K + e] = S[e];
   for(int e = 0; e < K; ++e) lam_out[size_t(i) * K + e] = lam[e];
+}
+
+// No eigenvalue of a local solver (the sums lambda_{i_0} + .. + lambda_{i_{d-1}}
+// over the directions) may be nearly zero: the check of
+// TensorProductMatrix::get_eigenvalues (tensor_product_matrix.h:357-371), taken
+// relative to the largest sum.  One thread per (component block, patch) row.
+template<int DIM, int K>
+__global__ void __launch_bounds__(64)
+k_check_eigenvalue_sums(const uint32_t *__restrict__ fac, const double *__restrict__ lam,
+                        const int nrows, int *__restrict__ status)
+{
+  const int r = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(r >= nrows) return;
+  const double *l[3];
+  for(int d = 0; d < DIM; ++d) l[d] = lam + size_t(fac[size_t(r) * DIM + d]) * K;
+  double big = 0.0;
+  for(int d = 0; d < DIM; ++d) big += fmax(fabs(l[d][0]), fabs(l[d][K - 1]));
+  double small = 1e300;
+  int idx[3] = {0, 0, 0};
+  for(int it = 0; it < cpow(K, DIM); ++it)
+  {
+    double s = 0.0;
+    for(int d = 0; d < DIM; ++d) s += l[d][idx[d]];
+    small = fmin(small, fabs(s));
+    for(int d = 0; d < DIM; ++d)
+    {
+      if(++idx[d] < K) break;
+      idx[d] = 0;
+    }
+  }
+  if(!(small > 1e-12 * big)) status[0] = 1;
 }
 
 // Surrogate lengths (A9, mapping_info.templates.h:42-240): per patch, direction

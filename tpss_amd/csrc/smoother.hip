@@ -86,6 +86,7 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int colo
   ls->K = 2 * lv->np;
   ls->n = n;
   ls->coloring = coloring;
+  ls->ncomp = lv->ncomp;
   ls->n_colors = n_colors_of(dim, coloring);
   const int P = ipow(n - 1, dim);
   ls->P = P;
@@ -160,6 +161,7 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int colo
     for(size_t i = 0; i < ls->fac.size(); ++i) ls->fac[i] = uint32_t(i);
     ls->prob_h = hs;
     ls->prob_b = bd;
+    ls->prob_coef.assign(size_t(ls->nfac) * 2, 1.0);
   }
   else
   {
@@ -172,26 +174,39 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int colo
         bd[(size_t(jp) * dim + d) * 2 + 0] = vd == 1;
         bd[(size_t(jp) * dim + d) * 2 + 1] = vd == n - 1;
       }
-    // deduplicate identical 1D problems
-    std::map<std::tuple<double, double, int, int>, uint32_t> uniq;
-    ls->fac.resize(size_t(P) * dim);
-    for(size_t i = 0; i < ls->fac.size(); ++i)
-    {
-      auto key = std::make_tuple(hs[2 * i], hs[2 * i + 1], int(bd[2 * i]), int(bd[2 * i + 1]));
-      auto it = uniq.find(key);
-      if(it == uniq.end())
+    // deduplicate identical 1D problems; with component blocks (elasticity) the
+    // 1D matrix of block c in direction d also depends on whether c == d
+    std::map<std::tuple<double, double, int, int, double, double>, uint32_t> uniq;
+    const size_t rows = size_t(P) * dim;
+    ls->fac.resize(rows * ls->ncomp);
+    for(int c = 0; c < ls->ncomp; ++c)
+      for(size_t i = 0; i < rows; ++i)
       {
-        const uint32_t id = uint32_t(uniq.size());
-        uniq.emplace(key, id);
-        ls->prob_h.push_back(hs[2 * i]);
-        ls->prob_h.push_back(hs[2 * i + 1]);
-        ls->prob_b.push_back(bd[2 * i]);
-        ls->prob_b.push_back(bd[2 * i + 1]);
-        ls->fac[i] = id;
+        const int d = int(i % dim);
+        double a_cons = 1.0, a_pen = 1.0;
+        if(ls->ncomp > 1)
+        {
+          a_cons = 2.0 * lv->mu * (c == d ? 1.0 : 0.5) + (c == d ? lv->lambda : 0.0);
+          a_pen = 2.0 * lv->mu + (c == d ? lv->lambda : 0.0);
+        }
+        auto key = std::make_tuple(hs[2 * i], hs[2 * i + 1], int(bd[2 * i]), int(bd[2 * i + 1]),
+                                   a_cons, a_pen);
+        auto it = uniq.find(key);
+        if(it == uniq.end())
+        {
+          const uint32_t id = uint32_t(uniq.size());
+          uniq.emplace(key, id);
+          ls->prob_h.push_back(hs[2 * i]);
+          ls->prob_h.push_back(hs[2 * i + 1]);
+          ls->prob_b.push_back(bd[2 * i]);
+          ls->prob_b.push_back(bd[2 * i + 1]);
+          ls->prob_coef.push_back(a_cons);
+          ls->prob_coef.push_back(a_pen);
+          ls->fac[c * rows + i] = id;
+        }
+        else
+          ls->fac[c * rows + i] = it->second;
       }
-      else
-        ls->fac[i] = it->second;
-    }
     ls->nfac = int(uniq.size());
   }
   ls->h_host = hs;
@@ -202,23 +217,30 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int colo
   ls->S64.resize(size_t(nf) * K * K);
   ls->lam64.resize(size_t(nf) * K);
   {
-    DeviceArray<double> d_h;
+    DeviceArray<double> d_h, d_coef;
     DeviceArray<uint8_t> d_b;
     d_h.upload(ls->prob_h);
     d_b.upload(ls->prob_b);
-    DeviceArray<int> d_status(1);
-    TPSS_HIP(hipMemsetAsync(d_status.ptr, 0, sizeof(int), s));
-    dispatch_dim_np(dim, lv->np, [&](auto, auto NPc) {
-      constexpr int NP = decltype(NPc)::value;
+    d_coef.upload(ls->prob_coef);
+    DeviceArray<int> d_status(2);
+    TPSS_HIP(hipMemsetAsync(d_status.ptr, 0, 2 * sizeof(int), s));
+    const int nrows = P * ls->ncomp;
+    dispatch_dim_np(dim, lv->np, [&](auto D, auto NPc) {
+      constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
       const auto sh = make_shape<NP>(NP - 1);
       hipLaunchKernelGGL((k_setup_fd<NP>), dim3(blocks_for(nf, 64)), dim3(64), 0, s, d_h.ptr,
-                         d_b.ptr, sh, NP - 1, lv->ip, nf, ls->S64.ptr, ls->lam64.ptr, d_status.ptr);
+                         d_b.ptr, d_coef.ptr, sh, NP - 1, lv->ip, nf, ls->S64.ptr, ls->lam64.ptr,
+                         d_status.ptr);
+      TPSS_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_check_eigenvalue_sums<DIM, 2 * NP>), dim3(blocks_for(nrows, 64)),
+                         dim3(64), 0, s, ls->d_fac.ptr, ls->lam64.ptr, nrows, d_status.ptr + 1);
       TPSS_HIP(hipGetLastError());
     });
-    int status = 0;
-    TPSS_HIP(hipMemcpyAsync(&status, d_status.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    int status[2] = {0, 0};
+    TPSS_HIP(hipMemcpyAsync(status, d_status.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     TPSS_HIP(hipStreamSynchronize(s));
-    if(status) throw Error(TPSS_E_SINGULAR, "local solver setup: mass matrix not s.p.d. or zero eigenvalue");
+    if(status[0]) throw Error(TPSS_E_SINGULAR, "local solver setup: mass matrix not s.p.d.");
+    if(status[1]) throw Error(TPSS_E_SINGULAR, "local solver setup: has zero eigenvalues");
   }
   ls->S32.resize(ls->S64.n);
   ls->lam32.resize(ls->lam64.n);
@@ -256,12 +278,19 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int colo
 
 // ---------------------------------------------------------------------------- apply
 
+// one launch over the patches [begin, end) of component block comp: gather,
+// fast-diagonalization inverse, damping, scatter (mode 0: scatter-add into dst;
+// mode 1: write the additive slot buffer)
 template<typename VT, typename CT>
 void patch_pass(tpss_smoother *sm, const VT *src, VT *dst, CT *ubuf, int begin, int end, int mode,
-                hipStream_t s)
+                hipStream_t s, int comp = 0)
 {
   const LocalSolvers &ls = *sm->ls;
   if(end <= begin) return;
+  const int64_t off = comp * sm->level->Ns;
+  src += off;
+  if(dst) dst += off;
+  const uint32_t *fac = ls.d_fac.ptr + size_t(comp) * ls.P * ls.dim;
   const CT *S = std::is_same<CT, double>::value ? (const CT *)ls.S64.ptr : (const CT *)ls.S32.ptr;
   const CT *lam =
     std::is_same<CT, double>::value ? (const CT *)ls.lam64.ptr : (const CT *)ls.lam32.ptr;
@@ -274,10 +303,10 @@ void patch_pass(tpss_smoother *sm, const VT *src, VT *dst, CT *ubuf, int begin, 
     const unsigned threads = unsigned(PS::THREADS);
     if(mode == 0)
       hipLaunchKernelGGL((k_patch_fd<DIM, K, VT, CT, 0>), dim3(grid), dim3(threads), 0, s, src,
-                         dst, ubuf, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end, ls.n);
+                         dst, ubuf, ls.d_vtx.ptr, fac, S, lam, om, relax, begin, end, ls.n);
     else
       hipLaunchKernelGGL((k_patch_fd<DIM, K, VT, CT, 1>), dim3(grid), dim3(threads), 0, s, src,
-                         dst, ubuf, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end, ls.n);
+                         dst, ubuf, ls.d_vtx.ptr, fac, S, lam, om, relax, begin, end, ls.n);
     TPSS_HIP(hipGetLastError());
   });
 }
@@ -327,18 +356,19 @@ void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, b
 
 template<typename VT, typename CT>
 void avp_sum(tpss_smoother *sm, const CT *ubuf, VT *dst, bool accumulate, double alpha,
-             hipStream_t s)
+             hipStream_t s, int comp = 0)
 {
   const tpss_level *lv = sm->level;
+  dst += comp * lv->Ns;
   dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
     constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
-    const unsigned g = blocks_for(lv->N, 256);
+    const unsigned g = blocks_for(lv->Ns, 256);
     if(accumulate)
       hipLaunchKernelGGL((k_avp_sum<DIM, NP, VT, CT, true>), dim3(g), dim3(256), 0, s, ubuf, dst,
-                         lv->n, lv->N, alpha);
+                         lv->n, lv->Ns, alpha);
     else
       hipLaunchKernelGGL((k_avp_sum<DIM, NP, VT, CT, false>), dim3(g), dim3(256), 0, s, ubuf,
-                         dst, lv->n, lv->N, alpha);
+                         dst, lv->n, lv->Ns, alpha);
     TPSS_HIP(hipGetLastError());
   });
 }
@@ -398,12 +428,15 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
   SectionTimer step_timer(sm, 4, s);
   if(sm->desc.variant == TPSS_ADDITIVE)
   {
-    sm->ubuf.resize(size_t(lv->N) * (1 << lv->dim) * sizeof(CT));
+    sm->ubuf.resize(size_t(lv->Ns) * (1 << lv->dim) * sizeof(CT));
     CT *ub = (CT *)sm->ubuf.ptr;
     {
       SectionTimer t(sm, 1, s);
-      patch_pass<VT, CT>(sm, src, nullptr, ub, 0, ls.P, 1, s);
-      avp_sum<VT, CT>(sm, ub, dst, accumulate, accumulate ? alpha : 1.0, s);
+      for(int c = 0; c < ls.ncomp; ++c)
+      {
+        patch_pass<VT, CT>(sm, src, nullptr, ub, 0, ls.P, 1, s, c);
+        avp_sum<VT, CT>(sm, ub, dst, accumulate, accumulate ? alpha : 1.0, s, c);
+      }
     }
     return;
   }
@@ -414,7 +447,7 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
     sm->tmp2.resize(size_t(lv->N) * sizeof(VT));
     out = (VT *)sm->tmp2.ptr;
   }
-  if(!lv->distorted && ls.coloring == TPSS_COLOR_REDBLACK && sm->fused_mvp)
+  if(!lv->distorted && lv->ncomp == 1 && ls.coloring == TPSS_COLOR_REDBLACK && sm->fused_mvp)
   {
     // cells not yet reached by a color are read as neighbours: start from zero
     fill_zero(out, lv->vec_prec, lv->N, s);
@@ -441,7 +474,8 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
       r = res;
     }
     SectionTimer t(sm, 1, s);
-    patch_pass<VT, CT>(sm, r, out, nullptr, ls.color_ptr[c], ls.color_ptr[c + 1], 0, s);
+    for(int comp = 0; comp < ls.ncomp; ++comp)
+      patch_pass<VT, CT>(sm, r, out, nullptr, ls.color_ptr[c], ls.color_ptr[c + 1], 0, s, comp);
   }
   if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
 }
@@ -454,7 +488,7 @@ void precond_vmult_bf16(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s
 {
   LocalSolvers &ls = *sm->ls;
   const tpss_level *lv = sm->level;
-  if(sm->desc.variant != TPSS_MULTIPLICATIVE || lv->distorted ||
+  if(sm->desc.variant != TPSS_MULTIPLICATIVE || lv->distorted || lv->ncomp > 1 ||
      ls.coloring != TPSS_COLOR_REDBLACK || ls.dim != 3 || ls.K != 8)
     throw Error(TPSS_E_UNSUPPORTED,
                 "BF16-MFMA patch solves: multiplicative, red-black, Cartesian 3D Q3 only");
@@ -574,6 +608,8 @@ int tpss_smoother_create(tpss_level *level, const tpss_smoother_desc *desc, tpss
     validate_desc(desc);
     if(level->vec_prec == TPSS_F32 && desc->local_precision == TPSS_F64)
       throw Error(TPSS_E_UNSUPPORTED, "local precision F64 needs F64 level vectors");
+    if(level->ncomp > 1 && desc->local_precision == TPSS_BF16)
+      throw Error(TPSS_E_UNSUPPORTED, "linear elasticity: local precision F64 or F32");
     auto sm = std::make_unique<tpss_smoother>();
     sm->level = level;
     sm->desc = *desc;
@@ -673,7 +709,7 @@ int tpss_smoother_info(const tpss_smoother *sm, int *n_patches, int *n_colors, i
     TPSS_REQUIRE(sm, "null argument");
     if(n_patches) *n_patches = sm->ls->P;
     if(n_colors) *n_colors = sm->ls->n_colors;
-    if(n_dofs_per_patch) *n_dofs_per_patch = ipow(sm->ls->K, sm->ls->dim);
+    if(n_dofs_per_patch) *n_dofs_per_patch = sm->ls->ncomp * ipow(sm->ls->K, sm->ls->dim);
   });
 }
 
@@ -773,6 +809,19 @@ void patch_dof_indices_device(const LocalSolvers &ls, uint32_t *host_out)
                        nullptr, ls.d_vtx.ptr, ls.P, ls.n, d_out.ptr);
     TPSS_HIP(hipGetLastError());
   });
-  TPSS_HIP(hipMemcpy(host_out, d_out.ptr, size_t(total) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if(ls.ncomp == 1)
+  {
+    TPSS_HIP(hipMemcpy(host_out, d_out.ptr, size_t(total) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return;
+  }
+  // component blocks juxtaposed per patch (PatchTransferBlock, patch_transfer.h:340-354)
+  std::vector<uint32_t> base(static_cast<size_t>(total));
+  TPSS_HIP(hipMemcpy(base.data(), d_out.ptr, size_t(total) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  const int ndp = ipow(ls.K, ls.dim);
+  const int64_t Ns = int64_t(ipow(ls.n, ls.dim)) * ipow(ls.np, ls.dim);
+  for(int64_t j = 0; j < ls.P; ++j)
+    for(int c = 0; c < ls.ncomp; ++c)
+      for(int i = 0; i < ndp; ++i)
+        host_out[(j * ls.ncomp + c) * ndp + i] = uint32_t(base[j * ndp + i] + c * Ns);
 }
 }  // namespace tpss

@@ -16,15 +16,17 @@ namespace tpss
 struct LocalSolvers
 {
   int dim = 0, p = 0, np = 0, K = 0, n = 0, P = 0, n_colors = 0, coloring = 0;
+  int ncomp = 1;                   // component blocks (block-diagonal local solvers)
   std::vector<int> color_ptr;
   std::vector<uint32_t> vtx;       // packed vertex per patch
   std::vector<int> vtx_unpacked;   // P * dim
   DeviceArray<uint32_t> d_vtx;
-  std::vector<uint32_t> fac;       // P * dim rows of the factor table
+  std::vector<uint32_t> fac;       // ncomp * P * dim: row (c * P + patch) * dim + direction
   DeviceArray<uint32_t> d_fac;
   int nfac = 0;
   std::vector<double> prob_h;      // nfac * 2
   std::vector<uint8_t> prob_b;     // nfac * 2
+  std::vector<double> prob_coef;   // nfac * 2: (a_cons, a_pen) of the 1D matrix
   DeviceArray<double> S64, lam64;  // nfac * K * K, nfac * K
   DeviceArray<float> S32, lam32;
   std::vector<double> h_host;      // P * dim * 2 surrogate lengths
