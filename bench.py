@@ -38,6 +38,13 @@ def parse():
     ap.add_argument("--no-gmg", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the distorted-mesh GMG-CG with learned vs fixed damping")
+    ap.add_argument("--no-elasticity", action="store_true",
+                    help="skip the 3D DG Q2 linear elasticity smoother and GMG-CG")
+    ap.add_argument("--variants", default="f64,f32,bf16",
+                    help="smoother variants to time besides f32 (comma list of f64, bf16)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
+                    help="only run 5 FP64 -> FP32 conversions of N values (8N bytes read, 4N "
+                         "written): the reference kernel for scaling HBM counter readings")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=2)
     return ap.parse_args()
@@ -91,6 +98,44 @@ def time_smoother(t, lv, sm, steps, warmup, precision):
     return (t1 - t0) / steps, sections
 
 
+def elasticity_section(t, args, n=64, p=2, mu=1.0, lam=10.0, ip=10.0):
+    """3D linear elasticity, SIPG DG Q2 (3 x 27 DoFs per cell) on n^3 cells, Lame mu = 1,
+    lambda = 10, ip 10: block-diagonal vertex-patch smoother steps (648 patch-DoFs) and
+    GMG-CG to 1e-8 (FP64 and FP32 V-cycle), seeded U[-1,1] body-force vector."""
+    out = {"workload": f"3D SIPG DG Q{p} linear elasticity {n}^3 cells, mu {mu}, lambda {lam}, "
+                       f"ip {ip}, block-diagonal vertex patches"}
+    steps = max(2, args.steps // 4)
+    for name, variant, relax, prec in (("mvp_f32", t.MULTIPLICATIVE, 1.0, t.F32),
+                                       ("avp_f32", t.ADDITIVE, 0.15, t.F32),
+                                       ("mvp_f64", t.MULTIPLICATIVE, 1.0, t.F64)):
+        lv = t.Level(3, p, n, ip_factor=ip, precision=prec, elasticity=(mu, lam))
+        sm = t.SchwarzSmoother(lv, variant=variant, relaxation=relax, local_precision=prec)
+        sec, sections = time_smoother(t, lv, sm, steps, 1, prec)
+        apply_t, apply_n = sections["[SchwarzPrecond] Apply inverses:"]
+        res_t, res_n = sections["[SchwarzPrecond] Update residuals:"]
+        pd = sm.n_patches * sm.n_dofs_per_patch
+        out[name] = {"ms_per_step": 1e3 * sec, "patch_dofs_per_s": pd / sec,
+                     "apply_inverses_ms_per_step": 1e3 * apply_t / steps,
+                     "update_residuals_ms_per_step": 1e3 * res_t / steps,
+                     "n_dofs": lv.n_dofs, "n_patches": sm.n_patches,
+                     "patch_dofs": sm.n_dofs_per_patch}
+        del sm, lv
+    if not args.no_gmg:
+        for name, mp in (("gmg_cg_f64", t.F64), ("gmg_cg_mixed_f32", t.F32)):
+            g = t.GMGSolver(3, p, 2, int(np.log2(n // 2)) + 1, ip_factor=ip, mg_precision=mp,
+                            variant=t.MULTIPLICATIVE, relaxation=1.0, elasticity=(mu, lam))
+            b = t.DeviceVector(g.n_dofs)
+            t.fill_random(b, 2)
+            x = t.DeviceVector.zeros(g.n_dofs)
+            g.solve(x, b, rel_tol=1e-8, max_iter=100, raise_on_fail=False)  # warm-up solve
+            info, hist = g.solve(x, b, rel_tol=1e-8, max_iter=100, raise_on_fail=False)
+            out[name] = {"seconds": info.seconds, "iterations": info.iterations,
+                         "n_frac": info.n_frac, "reduction_rate": info.reduction_rate,
+                         "final_rel_residual": hist[-1] / hist[0]}
+            del g
+    return out
+
+
 def cpu_baseline(n, p, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as o
@@ -112,11 +157,28 @@ def main():
         sys.exit(2)
     import tpss_amd as t
 
+    if args.calibrate_copy:
+        import ctypes
+        src = t.DeviceVector(args.calibrate_copy, t.F64)
+        dst = t.DeviceVector(args.calibrate_copy, t.F32)
+        t.fill_random(src, 1)
+        for _ in range(5):
+            t._check(t.lib.tpss_vector_convert(dst._p, t.F32, src._p, t.F64,
+                                               ctypes.c_int64(args.calibrate_copy), None))
+        t.synchronize()
+        print(json.dumps({"calibration": "k_convert<float, double>", "n": args.calibrate_copy,
+                          "read_bytes": 8 * args.calibrate_copy,
+                          "write_bytes": 4 * args.calibrate_copy, "launches": 5}))
+        return
+
     n, p, dim = args.n, args.degree, 3
     ndc = (p + 1) ** dim
     results = {}
+    wanted = set(args.variants.split(","))
     for name, prec, lprec in (("f64", t.F64, t.F64), ("f32", t.F32, t.F32),
                               ("bf16", t.F32, t.BF16)):
+        if name != "f32" and name not in wanted:  # the FP32 variant is the headline value
+            continue
         lv = t.Level(dim, p, n, precision=prec)
         sm = t.SchwarzSmoother(lv, variant=t.MULTIPLICATIVE, coloring=t.COLOR_REDBLACK,
                                relaxation=1.0, local_precision=lprec)
@@ -184,6 +246,10 @@ def main():
                              "final_rel_residual": hist[-1] / hist[0]}
             del g
 
+    elast = {}
+    if not args.no_elasticity:
+        elast = elasticity_section(t, args)
+
     cpu = None
     if not args.no_cpu:
         rate, dt = cpu_baseline(n, p, args.cpu_steps, args.cpu_threads)
@@ -220,6 +286,7 @@ def main():
                             "kernels": r["kernels"]} for name, r in results.items()},
         "gmg_cg": gmg,
         "gmg_cg_distorted_learned_damping": learned,
+        "elasticity": elast,
     }
     print(json.dumps(out))
 

@@ -44,6 +44,13 @@ typedef enum { TPSS_F64 = 0, TPSS_F32 = 1, TPSS_BF16 = 2 } tpss_precision;
  * meshes; the Schwarz smoother uses the block-diagonal local solvers
  * (FD::MatrixIntegrator::assemble_subspace_inverses, :604-647). */
 typedef enum { TPSS_POISSON = 0, TPSS_ELASTICITY = 1 } tpss_equation;
+/* Local solvers of distorted patches: TPSS_LOCAL_SURROGATE = fast diagonalization
+ * of the Cartesian patch with surrogate lengths (MappingInfo::compute_average_scaling,
+ * mapping_info.templates.h:42-240, the reference's path); TPSS_LOCAL_FIT = fast
+ * diagonalization of the separable Kronecker fit (alternating least squares,
+ * rank-2 Kronecker projection per direction) of the exact local matrix
+ * R_j A R_j^T, initialised with the surrogate matrices (kroneckersvd.h:215-585). */
+typedef enum { TPSS_LOCAL_SURROGATE = 0, TPSS_LOCAL_FIT = 1 } tpss_local_solver;
 
 typedef struct tpss_level tpss_level;
 typedef struct tpss_smoother tpss_smoother;
@@ -80,6 +87,8 @@ typedef struct
   double damping_factor; /* Richardson omega_R */
   int n_steps;           /* Richardson steps per step() */
   int local_precision;   /* tpss_precision of the patch solves */
+  int local_solver;      /* tpss_local_solver (0 = surrogate lengths) */
+  int fit_sweeps;        /* ALS sweeps of TPSS_LOCAL_FIT (0 = 4) */
 } tpss_smoother_desc;
 
 typedef struct
@@ -136,6 +145,10 @@ int tpss_smoother_patch_vertices(const tpss_smoother *sm, int *out);
 int tpss_smoother_patch_dof_indices(const tpss_smoother *sm, uint32_t *out);
 /* surrogate lengths h[patch][direction][cell_no_1d] (mapping_info.h:126-130), host */
 int tpss_smoother_patch_h(const tpss_smoother *sm, double *out);
+/* relative Frobenius error ||A_j - fit_j|| / ||A_j|| of every patch's separable
+ * fit (TPSS_LOCAL_FIT; host double[n_patches], patches whose fitted mass matrix
+ * is not s.p.d. keep the surrogate factors and report -1) */
+int tpss_smoother_fit_errors(const tpss_smoother *sm, double *out);
 /* SchwarzPreconditioner::get_time_data (schwarz_preconditioner.h:87-199):
  * sections "Update residuals", "Apply inverses", "Compute inverses",
  * "Copy Locally Owned", "Step"; *n in: capacity, out: count. */

@@ -133,12 +133,15 @@ struct AdditionalData
   double damping_factor = 1.0;
   int n_steps = 1;
   int local_precision = -1;  // -1: the level's vector precision
+  int local_solver = TPSS_LOCAL_SURROGATE;
+  int fit_sweeps = 0;
 
   tpss_smoother_desc desc(int vec_prec) const
   {
     return tpss_smoother_desc{variant,       coloring,       relaxation,
                               int(reverse),  int(symmetrized), damping_factor,
-                              n_steps,       local_precision < 0 ? vec_prec : local_precision};
+                              n_steps,       local_precision < 0 ? vec_prec : local_precision,
+                              local_solver,  fit_sweeps};
   }
 };
 

@@ -14,6 +14,7 @@ _LIB = os.path.join(_HERE, "libtpss_oracle.so")
 
 ADDITIVE, MULTIPLICATIVE = 0, 1
 COLOR_NONE, COLOR_TILED, COLOR_REDBLACK = 0, 1, 2
+LOCAL_SURROGATE, LOCAL_FIT = 0, 1
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -49,6 +50,11 @@ def _load():
                                                    ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                    ctypes.c_double, ctypes.c_int]),
         "or_n_components": (ctypes.c_int, [ctypes.c_void_p]),
+        "or_set_local_solver": (None, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+        "or_fit_errors": (None, [ctypes.c_void_p, _D]),
+        "or_fit_fallbacks": (ctypes.c_int, [ctypes.c_void_p]),
+        "or_patch_local_matrix": (None, [ctypes.c_void_p, ctypes.c_int, _D]),
+        "or_separable_fit": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, _D, _D, _D, ctypes.c_int]),
         "or_destroy": (None, [ctypes.c_void_p]),
         "or_set_vertices": (ctypes.c_int, [ctypes.c_void_p, _D]),
         "or_setup": (ctypes.c_int, [ctypes.c_void_p]),
@@ -145,6 +151,16 @@ def patch_matrices_1d_coef(p, h, bdry_left, bdry_right, ip, a_cons, a_pen):
     return M, L
 
 
+def separable_fit(dim, A, M, L, sweeps):
+    """ALS separable fit of A (k^dim x k^dim); M, L: (dim, k, k) initial guess."""
+    k = M.shape[-1]
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    M = np.ascontiguousarray(M, dtype=np.float64).copy()
+    L = np.ascontiguousarray(L, dtype=np.float64).copy()
+    err = lib.or_separable_fit(dim, k, dp(A), dp(M), dp(L), sweeps)
+    return M, L, err
+
+
 def gen_eig(A, B):
     k = A.shape[0]
     A = np.ascontiguousarray(A, dtype=np.float64)
@@ -201,7 +217,8 @@ class Smoother:
 
     def __init__(self, dim, p, n, ip=1.0, variant=ADDITIVE, coloring=COLOR_NONE,
                  relaxation=1.0, reverse=False, symmetrized=False, damping=1.0,
-                 n_steps=1, vertices=None, threads=1, elasticity=None):
+                 n_steps=1, vertices=None, threads=1, elasticity=None, local_solver=LOCAL_SURROGATE,
+                 fit_sweeps=4):
         """elasticity=(mu, lambda) selects the linear elasticity operator with
         dim components and block-diagonal local solvers."""
         self.dim, self.p, self.n = dim, p, n
@@ -220,6 +237,8 @@ class Smoother:
             lib.or_set_vertices(self.h, dp(self._v))
         if threads > 1:
             lib.or_set_threads(self.h, threads)
+        if local_solver != LOCAL_SURROGATE:
+            lib.or_set_local_solver(self.h, local_solver, fit_sweeps)
         st = lib.or_setup(self.h)
         if st:
             raise np.linalg.LinAlgError("local solver setup failed")
@@ -263,6 +282,19 @@ class Smoother:
         if self.ncomp == 1:
             return S[:, 0], lam[:, 0]
         return S, lam
+
+    def fit_errors(self):
+        out = np.zeros(self.P)
+        lib.or_fit_errors(self.h, dp(out))
+        return out
+
+    def fit_fallbacks(self):
+        return lib.or_fit_fallbacks(self.h)
+
+    def patch_local_matrix(self, patch):
+        A = np.zeros((self.ndp, self.ndp))
+        lib.or_patch_local_matrix(self.h, int(patch), dp(A))
+        return A
 
     def color_sequence(self, transpose=False):
         out = np.zeros(2 * self.n_colors, dtype=np.int32)

@@ -472,6 +472,9 @@ struct or_ctx
   double Fmm[OR_MAXNP * OR_MAXNP], Fmp[OR_MAXNP * OR_MAXNP];
   double Fpm[OR_MAXNP * OR_MAXNP], Fpp[OR_MAXNP * OR_MAXNP];
   double Bl[OR_MAXNP * OR_MAXNP], Br[OR_MAXNP * OR_MAXNP];
+  /* local solvers: surrogate-length FD (0) or separable fit of the exact local matrix (1) */
+  int local_solver, fit_sweeps, n_fit_fallback;
+  double *fit_err; /* P: relative Frobenius error of the fitted separable matrix */
   int ready;
 };
 
@@ -565,9 +568,23 @@ or_ctx *or_create_elasticity(int dim, int p, int n, double ip, double mu, double
 
 int or_n_components(const or_ctx *c) { return c->ncomp; }
 
+void or_set_local_solver(or_ctx *c, int kind, int sweeps)
+{
+  c->local_solver = kind;
+  c->fit_sweeps = sweeps > 0 ? sweeps : 4;
+}
+
+void or_fit_errors(const or_ctx *c, double *out)
+{
+  for(int j = 0; j < c->P; ++j) out[j] = c->fit_err ? c->fit_err[j] : 0.0;
+}
+
+int or_fit_fallbacks(const or_ctx *c) { return c->n_fit_fallback; }
+
 void or_destroy(or_ctx *c)
 {
   if(!c) return;
+  free(c->fit_err);
   free(c->vertices);
   free(c->color_ptr);
   free(c->vtx);
@@ -794,6 +811,37 @@ int or_setup(or_ctx *c)
     }
   }
   build_operator_1d(c);
+  if(c->local_solver == OR_LOCAL_FIT && c->ncomp == 1 && !status)
+  {
+    /* separable fit of the exact local matrix R_j A R_j^T, initialised with
+     * the surrogate-length matrices; FD factors of the fit */
+    const int ndp = c->ndp;
+    double *A = (double *)malloc(sizeof(double) * ndp * ndp);
+    free(c->fit_err);
+    c->fit_err = (double *)calloc(P, sizeof(double));
+    c->n_fit_fallback = 0;
+    for(int j = 0; j < P; ++j)
+    {
+      const int *v = c->vtx + j * dim;
+      double M[3 * OR_MAXK * OR_MAXK], L[3 * OR_MAXK * OR_MAXK];
+      for(int d = 0; d < dim; ++d)
+        or_patch_matrices_1d(c->p, c->h + (j * dim + d) * 2, v[d] - 1 == 0, v[d] + 1 == n, c->ip,
+                             M + d * k * k, L + d * k * k);
+      or_patch_local_matrix(c, j, A);
+      c->fit_err[j] = or_separable_fit(dim, k, A, M, L, c->fit_sweeps);
+      double S[3 * OR_MAXK * OR_MAXK], lam[3 * OR_MAXK];
+      int bad = 0;
+      for(int d = 0; d < dim; ++d) bad |= or_gen_eig(k, L + d * k * k, M + d * k * k, S + d * k * k, lam + d * k);
+      if(bad)
+      {
+        ++c->n_fit_fallback; /* fitted mass not s.p.d.: keep the surrogate factors */
+        continue;
+      }
+      memcpy(c->S + (long)j * dim * k * k, S, sizeof(double) * dim * k * k);
+      memcpy(c->lam + (long)j * dim * k, lam, sizeof(double) * dim * k);
+    }
+    free(A);
+  }
   c->ready = 1;
   return status;
 }
@@ -939,6 +987,133 @@ void or_operator_vmult(const or_ctx *c, double *dst, const double *src)
   if(c->ncomp > 1) operator_elasticity(c, dst, src);
   else if(c->vertices) operator_distorted(c, dst, src);
   else operator_cartesian(c, dst, src);
+}
+
+/* ---------------- exact local matrices and their separable fit ----------- */
+
+/* A_j = R_j A R_j^T (row-major ndp x ndp), column by column from the global
+ * operator applied to unit vectors: the patch matrix the reference's
+ * TensorProductMatrix "basic" state and the Kronecker SVD start from
+ * (tensor_product_matrix.h:136-304, kroneckersvd.h:215-296). */
+void or_patch_local_matrix(const or_ctx *c, int patch, double *A)
+{
+  const int ndp = c->ndp;
+  uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * ndp * c->ncomp);
+  double *x = (double *)calloc(c->N, sizeof(double));
+  double *y = (double *)malloc(sizeof(double) * c->N);
+  patch_dofs(c, c->vtx + patch * c->dim, idx);
+  for(int col = 0; col < ndp; ++col)
+  {
+    x[idx[col]] = 1.0;
+    or_operator_vmult(c, y, x);
+    x[idx[col]] = 0.0;
+    for(int row = 0; row < ndp; ++row) A[(long)row * ndp + col] = y[idx[row]];
+  }
+  free(idx);
+  free(x);
+  free(y);
+}
+
+/*
+ * Separable fit  A ~ sum_d M_{dim-1} x .. x L_d x .. x M_0  (dimension 0
+ * rightmost) by alternating least squares in the Frobenius norm: for each
+ * direction d in turn the pair (L_d, M_d) solves the rank-2 Kronecker
+ * least-squares problem
+ *   min || A - L_d (x)_d R1 - M_d (x)_d R2 ||_F,
+ *   R1 = (x)_{e != d} M_e,  R2 = sum_{e != d} (x)_{f != d} (L_f if f == e else M_f),
+ * one 2x2 normal-equation system per entry (i_d, j_d).  This is the CP-ALS of
+ * the reference (kroneckersvd.h:501-585) with the factors tied to the
+ * separable form, so that the fit stays invertible by fast diagonalization
+ * (TensorProductMatrixSymmetricSum).  M, L: dim blocks of k x k, in: initial
+ * guess, out: fit.  Returns ||A - fit||_F / ||A||_F.
+ */
+double or_separable_fit(int dim, int k, const double *A, double *M, double *L, int sweeps)
+{
+  const int ndp = ipow(k, dim);
+  double nM[3], nL[3], ML[3];
+  for(int s = 0; s < sweeps; ++s)
+    for(int d = 0; d < dim; ++d)
+    {
+      for(int e = 0; e < dim; ++e)
+      {
+        nM[e] = nL[e] = ML[e] = 0.0;
+        for(int q = 0; q < k * k; ++q)
+        {
+          nM[e] += M[e * k * k + q] * M[e * k * k + q];
+          nL[e] += L[e * k * k + q] * L[e * k * k + q];
+          ML[e] += M[e * k * k + q] * L[e * k * k + q];
+        }
+      }
+      /* Gram matrix of (R1, R2) */
+      double g11 = 1.0, g12 = 0.0, g22 = 0.0;
+      for(int e = 0; e < dim; ++e)
+        if(e != d) g11 *= nM[e];
+      for(int e = 0; e < dim; ++e)
+      {
+        if(e == d) continue;
+        double t12 = ML[e], t22 = nL[e];
+        for(int f = 0; f < dim; ++f)
+          if(f != d && f != e) { t12 *= nM[f]; t22 *= nM[f]; }
+        g12 += t12;
+        g22 += t22;
+        for(int e2 = 0; e2 < dim; ++e2)
+        {
+          if(e2 == d || e2 == e) continue;
+          double t = ML[e] * ML[e2];
+          for(int f = 0; f < dim; ++f)
+            if(f != d && f != e && f != e2) t *= nM[f];
+          g22 += t;
+        }
+      }
+      double T1[OR_MAXK * OR_MAXK], T2[OR_MAXK * OR_MAXK];
+      memset(T1, 0, sizeof(T1));
+      memset(T2, 0, sizeof(T2));
+      for(long r = 0; r < ndp; ++r)
+        for(long col = 0; col < ndp; ++col)
+        {
+          int ri[3], ci[3];
+          long tr = r, tc = col;
+          for(int e = 0; e < dim; ++e) { ri[e] = (int)(tr % k); tr /= k; ci[e] = (int)(tc % k); tc /= k; }
+          double w1 = 1.0, w2 = 0.0;
+          for(int e = 0; e < dim; ++e)
+          {
+            if(e == d) continue;
+            const double m = M[e * k * k + ri[e] * k + ci[e]], l = L[e * k * k + ri[e] * k + ci[e]];
+            w2 = w2 * m + w1 * l;
+            w1 *= m;
+          }
+          const double a = A[r * ndp + col];
+          T1[ri[d] * k + ci[d]] += a * w1;
+          T2[ri[d] * k + ci[d]] += a * w2;
+        }
+      const double det = g11 * g22 - g12 * g12;
+      for(int q = 0; q < k * k; ++q)
+      {
+        L[d * k * k + q] = (g22 * T1[q] - g12 * T2[q]) / det;
+        M[d * k * k + q] = (g11 * T2[q] - g12 * T1[q]) / det;
+      }
+    }
+  /* relative Frobenius error of the fit */
+  double num = 0.0, den = 0.0;
+  for(long r = 0; r < ndp; ++r)
+    for(long col = 0; col < ndp; ++col)
+    {
+      int ri[3], ci[3];
+      long tr = r, tc = col;
+      for(int e = 0; e < dim; ++e) { ri[e] = (int)(tr % k); tr /= k; ci[e] = (int)(tc % k); tc /= k; }
+      double fit = 0.0;
+      for(int d = 0; d < dim; ++d)
+      {
+        double t = 1.0;
+        for(int e = 0; e < dim; ++e)
+          t *= (e == d ? L : M)[e * k * k + ri[e] * k + ci[e]];
+        fit += t;
+      }
+      const double a = A[r * ndp + col];
+      num += (a - fit) * (a - fit);
+      den += a * a;
+    }
+  return sqrt(num / den);
 }
 
 /* ---------------- Schwarz preconditioner (A1, A2, A12) ------------------- */

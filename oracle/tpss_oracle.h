@@ -27,6 +27,7 @@ extern "C" {
 
 enum { OR_ADDITIVE = 0, OR_MULTIPLICATIVE = 1 };
 enum { OR_COLOR_NONE = 0, OR_COLOR_TILED = 1, OR_COLOR_REDBLACK = 2 };
+enum { OR_LOCAL_SURROGATE = 0, OR_LOCAL_FIT = 1 };
 
 /* ---- 1D building blocks ------------------------------------------------ */
 void or_gauss_legendre(int nq, double *x, double *w);
@@ -55,6 +56,14 @@ or_ctx *or_create_elasticity(int dim, int p, int n, double ip, double mu, double
                              int variant, int coloring, double relaxation, int reverse,
                              int symmetrized, double richardson_damping, int n_steps);
 int or_n_components(const or_ctx *c);
+/* local solvers: OR_LOCAL_SURROGATE (surrogate patch lengths, the reference's
+ * distorted-mesh path) or OR_LOCAL_FIT (separable ALS fit of the exact local
+ * matrix, `sweeps` sweeps); call before or_setup */
+void or_set_local_solver(or_ctx *c, int kind, int sweeps);
+void or_fit_errors(const or_ctx *c, double *out);
+int or_fit_fallbacks(const or_ctx *c);
+void or_patch_local_matrix(const or_ctx *c, int patch, double *A);
+double or_separable_fit(int dim, int k, const double *A, double *M, double *L, int sweeps);
 void or_destroy(or_ctx *c);
 int or_set_vertices(or_ctx *c, const double *coords);
 int or_setup(or_ctx *c);

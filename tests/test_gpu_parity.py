@@ -315,3 +315,25 @@ def test_gmg_bf16_reaches_tolerance():
     info, hist = g.solve(x, dev(b), rel_tol=1e-8)
     assert hist[-1] <= 1e-8 * hist[0]
     assert info.iterations <= 12
+
+
+@pytest.mark.parametrize("dim,p,n,rev,n_steps", [(2, 3, 9, False, 1), (3, 2, 6, True, 2),
+                                                 (3, 3, 5, False, 1)])
+def test_residual_free_richardson_step(dim, p, n, rev, n_steps):
+    """With omega_R = 1 the multiplicative step runs as color passes on (x, b)
+    without a residual update; it must match the reference's residual + sweep."""
+    import tpss_amd as t
+    lv, sm, ref = make(dim, p, n, t.MULTIPLICATIVE, reverse=rev, n_steps=n_steps)
+    b = o.splitmix_uniform(4, ref.N)
+    x0 = o.splitmix_uniform(5, ref.N)
+    expect = ref.step(x0.copy(), b)
+    d = dev(x0)
+    sm.step(d, dev(b))
+    assert rel_err(host(d), expect) < 1e-12
+    sm.set_fused_residual(False)
+    d = dev(x0)
+    sm.step(d, dev(b))
+    assert rel_err(host(d), expect) < 1e-12
+    sm.set_fused_residual(True)
+    sm.vmult(d, dev(b))  # from zero
+    assert rel_err(host(d), ref.smoother_vmult(b)) < 1e-12

@@ -23,6 +23,7 @@ ADDITIVE, MULTIPLICATIVE = 0, 1
 COLOR_NONE, COLOR_TILED, COLOR_REDBLACK = 0, 1, 2
 F64, F32, BF16 = 0, 1, 2
 POISSON, ELASTICITY = 0, 1
+LOCAL_SURROGATE, LOCAL_FIT = 0, 1
 
 #: TPSS::lookup_damping_factor (TPSS.h:120-163) for vertex patches
 DEFAULT_DAMPING = {(ADDITIVE, 2): 0.25, (ADDITIVE, 3): 0.15,
@@ -46,7 +47,8 @@ class SmootherDesc(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("coloring", ctypes.c_int),
                 ("relaxation", ctypes.c_double), ("reverse", ctypes.c_int),
                 ("symmetrized", ctypes.c_int), ("damping_factor", ctypes.c_double),
-                ("n_steps", ctypes.c_int), ("local_precision", ctypes.c_int)]
+                ("n_steps", ctypes.c_int), ("local_precision", ctypes.c_int),
+                ("local_solver", ctypes.c_int), ("fit_sweeps", ctypes.c_int)]
 
 
 class TimeInfo(ctypes.Structure):
@@ -76,6 +78,7 @@ EXPORTS = [
     "tpss_precond_vmult", "tpss_smoother_step", "tpss_smoother_vmult",
     "tpss_smoother_set_patch_damping", "tpss_smoother_info", "tpss_smoother_color_ptr",
     "tpss_smoother_patch_vertices", "tpss_smoother_patch_dof_indices", "tpss_smoother_patch_h",
+    "tpss_smoother_fit_errors",
     "tpss_smoother_get_timings", "tpss_smoother_set_timing", "tpss_smoother_set_check_finite",
     "tpss_smoother_set_fused_residual",
     "tpss_gmg_create", "tpss_gmg_destroy", "tpss_gmg_level", "tpss_gmg_smoother",
@@ -266,9 +269,10 @@ def _vertices_ptr(vertices):
 
 
 def _smoother_desc(variant, coloring, relaxation, reverse, symmetrized, damping_factor, n_steps,
-                   local_precision):
+                   local_precision, local_solver=LOCAL_SURROGATE, fit_sweeps=0):
     return SmootherDesc(variant, coloring, float(relaxation), int(reverse), int(symmetrized),
-                        float(damping_factor), int(n_steps), int(local_precision))
+                        float(damping_factor), int(n_steps), int(local_precision),
+                        int(local_solver), int(fit_sweeps))
 
 
 class Level:
@@ -320,7 +324,7 @@ class SchwarzPreconditioner:
 
     def __init__(self, level, variant=ADDITIVE, coloring=None, relaxation=None, reverse=False,
                  symmetrized=False, damping_factor=1.0, n_steps=1, local_precision=None,
-                 _handle=None):
+                 local_solver=LOCAL_SURROGATE, fit_sweeps=0, _handle=None):
         if coloring is None:
             coloring = COLOR_NONE if variant == ADDITIVE else COLOR_REDBLACK
         if relaxation is None:
@@ -329,7 +333,8 @@ class SchwarzPreconditioner:
             local_precision = level.precision
         self.level = level
         self.desc = _smoother_desc(variant, coloring, relaxation, reverse, symmetrized,
-                                   damping_factor, n_steps, local_precision)
+                                   damping_factor, n_steps, local_precision, local_solver,
+                                   fit_sweeps)
         if _handle is not None:
             self.handle, self._owned = _handle, False
         else:
@@ -429,6 +434,14 @@ class SchwarzPreconditioner:
         _check(lib.tpss_smoother_patch_h(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def fit_errors(self):
+        """Relative Frobenius error of each patch's separable fit (LOCAL_FIT)."""
+        import numpy as np
+        out = np.zeros(self.n_patches)
+        _check(lib.tpss_smoother_fit_errors(self.handle,
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     def __del__(self):
         if getattr(self, "_owned", False) and self.handle:
             lib.tpss_smoother_destroy(self.handle)
@@ -455,7 +468,7 @@ class GMGSolver:
     def __init__(self, dim, degree, n_cells_coarse, n_levels, ip_factor=1.0, vertices=None,
                  mg_precision=F64, variant=MULTIPLICATIVE, coloring=None, relaxation=None,
                  damping_factor=1.0, n_steps=1, local_precision=None, coarse_tol=1e-12,
-                 elasticity=None):
+                 elasticity=None, local_solver=LOCAL_SURROGATE, fit_sweeps=0):
         if coloring is None:
             coloring = COLOR_NONE if variant == ADDITIVE else COLOR_REDBLACK
         if relaxation is None:
@@ -464,7 +477,7 @@ class GMGSolver:
             local_precision = mg_precision
         self._varr, vp = _vertices_ptr(vertices)
         sd = _smoother_desc(variant, coloring, relaxation, False, False, damping_factor, n_steps,
-                            local_precision)
+                            local_precision, local_solver, fit_sweeps)
         eq, mu, lam = (POISSON, 0.0, 0.0) if elasticity is None else (ELASTICITY,) + tuple(
             float(v) for v in elasticity)
         self.desc = GMGDesc(dim, degree, n_cells_coarse, n_levels, float(ip_factor), vp,

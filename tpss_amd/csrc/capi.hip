@@ -180,6 +180,12 @@ void operator_apply_any(const tpss_level *lv, void *y, const void *x, const void
     operator_apply<float>(lv, (float *)y, (const float *)x, (const float *)b, cells, n_list, s);
 }
 
+void operator_apply_f64(const tpss_level *lv, double *y, const double *x, const uint32_t *cells,
+                        int64_t n_list, hipStream_t s)
+{
+  operator_apply<double>(lv, y, x, nullptr, cells, n_list, s);
+}
+
 size_t elem_size(int prec) { return prec == TPSS_F64 ? 8 : 4; }
 
 template<typename T>

@@ -237,7 +237,7 @@ void vcycle(tpss_gmg *g, int l, hipStream_t s)
   }
   // pre-smoothing from a zero initial guess: SmootherRichardson::vmult
   fill_zero(x, g->mg_prec, lv->N, s);
-  smoother_step_any(g->pre[l], x, b, s);
+  smoother_step_any(g->pre[l], x, b, s, true);
   operator_apply_any(lv, r, x, b, nullptr, 0, s);
   tpss_level *lc = g->levels[l - 1];
   const int st1 = tpss_transfer_restrict(lc, lv, g->vb[l - 1].ptr, r, s);

@@ -24,6 +24,70 @@ namespace tpss
 typedef short tpss_bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
 typedef float tpss_floatx4 __attribute__((__vector_size__(4 * sizeof(float))));
 
+// Contiguous rows of a cell (NP values along x) moved with 16-byte vector
+// accesses; the types only claim element alignment, so any element offset is
+// valid (global memory accesses need no 16-byte alignment on CDNA).
+typedef float tpss_f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double tpss_d2u __attribute__((ext_vector_type(2), aligned(8)));
+
+template<int NP, typename VT, typename CT>
+__device__ __forceinline__ void load_row(const VT *__restrict__ p, CT *out)
+{
+  if constexpr(std::is_same<VT, float>::value && NP % 4 == 0)
+  {
+#pragma unroll
+    for(int i = 0; i < NP; i += 4)
+    {
+      const tpss_f4u v = *reinterpret_cast<const tpss_f4u *>(p + i);
+      out[i] = CT(v.x), out[i + 1] = CT(v.y), out[i + 2] = CT(v.z), out[i + 3] = CT(v.w);
+    }
+  }
+  else if constexpr(std::is_same<VT, double>::value && NP % 2 == 0)
+  {
+#pragma unroll
+    for(int i = 0; i < NP; i += 2)
+    {
+      const tpss_d2u v = *reinterpret_cast<const tpss_d2u *>(p + i);
+      out[i] = CT(v.x), out[i + 1] = CT(v.y);
+    }
+  }
+  else
+  {
+#pragma unroll
+    for(int i = 0; i < NP; ++i) out[i] = CT(p[i]);
+  }
+}
+
+template<int NP, typename VT, typename CT>
+__device__ __forceinline__ void store_row(VT *__restrict__ p, const CT *in)
+{
+  if constexpr(std::is_same<VT, float>::value && NP % 4 == 0)
+  {
+#pragma unroll
+    for(int i = 0; i < NP; i += 4)
+    {
+      tpss_f4u v;
+      v.x = float(in[i]), v.y = float(in[i + 1]), v.z = float(in[i + 2]), v.w = float(in[i + 3]);
+      *reinterpret_cast<tpss_f4u *>(p + i) = v;
+    }
+  }
+  else if constexpr(std::is_same<VT, double>::value && NP % 2 == 0)
+  {
+#pragma unroll
+    for(int i = 0; i < NP; i += 2)
+    {
+      tpss_d2u v;
+      v.x = double(in[i]), v.y = double(in[i + 1]);
+      *reinterpret_cast<tpss_d2u *>(p + i) = v;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for(int i = 0; i < NP; ++i) p[i] = VT(in[i]);
+  }
+}
+
 // FP32 -> BF16 bits, round to nearest even (finite inputs)
 __device__ __forceinline__ short bf16_bits(float f)
 {
@@ -142,12 +206,18 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   const int by = ty >= NP ? 1 : 0, bz = tz >= NP ? 1 : 0;
   const int64_t nn = int64_t(n) * n;
 
+  // the x-line (ty, tz) of the patch: row lyz of the cells c_lo and c_lo + 1
+  const int64_t c_lo = patch_cell<DIM, NP>(v, n, 0, ty, tz);
+  const int lyz = patch_local<DIM, NP>(0, ty, tz);
   CT g[K], r[K], o[K];
+  if(valid)
+  {
+    load_row<NP>(b + c_lo * NDC + lyz, g);
+    load_row<NP>(b + (c_lo + 1) * NDC + lyz, g + NP);
+  }
+  else
 #pragma unroll
-  for(int i = 0; i < K; ++i)
-    g[i] = valid ? CT(b[patch_cell<DIM, NP>(v, n, i, ty, tz) * NDC + patch_local<DIM, NP>(i, ty, tz)])
-                 : CT(0);
-  __syncthreads();
+    for(int i = 0; i < K; ++i) g[i] = CT(0);
 
   if(!x_is_zero)
   {
@@ -160,29 +230,29 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
       t1[i] = 0;
       t2[i] = 0;
     }
-    const int64_t c_lo = patch_cell<DIM, NP>(v, n, 0, ty, tz);      // x-cell with b_0 = 0
-    const int lyz = patch_local<DIM, NP>(0, ty, tz);                 // local offset of the line
     if(valid && v[0] - 1 > 0)
     {
-      const VT *xl = x + (c_lo - 1) * NDC + lyz;
+      CT xl[NP];
+      load_row<NP>(x + (c_lo - 1) * NDC + lyz, xl);
 #pragma unroll
       for(int i = 0; i < NP; ++i)
       {
         CT s = 0;
 #pragma unroll
-        for(int jj = 0; jj < NP; ++jj) s += tab.Fpm[i * NP + jj] * CT(xl[jj]);
+        for(int jj = 0; jj < NP; ++jj) s += tab.Fpm[i * NP + jj] * xl[jj];
         z0[i] = s;
       }
     }
     if(valid && v[0] + 1 < n)
     {
-      const VT *xr = x + (c_lo + 2) * NDC + lyz;
+      CT xr[NP];
+      load_row<NP>(x + (c_lo + 2) * NDC + lyz, xr);
 #pragma unroll
       for(int i = 0; i < NP; ++i)
       {
         CT s = 0;
 #pragma unroll
-        for(int jj = 0; jj < NP; ++jj) s += tab.Fmp[i * NP + jj] * CT(xr[jj]);
+        for(int jj = 0; jj < NP; ++jj) s += tab.Fmp[i * NP + jj] * xr[jj];
         z0[NP + i] = s;
       }
     }
@@ -197,13 +267,17 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
 #pragma unroll
         for(int cx = 0; cx < 2; ++cx)
         {
-          const VT *xn = x + (c_lo + cx + shift) * NDC + NP * NP * lz;
+          // the x-y plane lz of the neighbour cell: NP rows, contiguous
+          CT xn[NP * NP];
+          const VT *pn = x + (c_lo + cx + shift) * NDC + NP * NP * lz;
+#pragma unroll
+          for(int jj = 0; jj < NP; ++jj) load_row<NP>(pn + NP * jj, xn + NP * jj);
 #pragma unroll
           for(int lx = 0; lx < NP; ++lx)
           {
             CT s = 0;
 #pragma unroll
-            for(int jj = 0; jj < NP; ++jj) s += F[ly * NP + jj] * CT(xn[lx + NP * jj]);
+            for(int jj = 0; jj < NP; ++jj) s += F[ly * NP + jj] * xn[lx + NP * jj];
             z1[cx * NP + lx] = s;
           }
         }
@@ -230,13 +304,17 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
 #pragma unroll
         for(int cx = 0; cx < 2; ++cx)
         {
-          const VT *xn = x + (c_lo + cx + shift) * NDC + NP * ly;
+          // the x-z plane ly of the neighbour cell: NP rows at stride NP^2
+          CT xn[NP * NP];
+          const VT *pn = x + (c_lo + cx + shift) * NDC + NP * ly;
+#pragma unroll
+          for(int kk = 0; kk < NP; ++kk) load_row<NP>(pn + NP * NP * kk, xn + NP * kk);
 #pragma unroll
           for(int lx = 0; lx < NP; ++lx)
           {
             CT s = 0;
 #pragma unroll
-            for(int kk = 0; kk < NP; ++kk) s += F[lz * NP + kk] * CT(xn[lx + NP * NP * kk]);
+            for(int kk = 0; kk < NP; ++kk) s += F[lz * NP + kk] * xn[lx + NP * kk];
             z2[cx * NP + lx] = s;
           }
         }
@@ -302,6 +380,9 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
     }
   }
 
+  // the factor tables in LDS are complete (without the coupling stage no
+  // barrier has passed since they were written)
+  if(x_is_zero) __syncthreads();
   if constexpr(FD == 1)
   {
     static_assert(DIM == 3 && K == 8 && PPB == 2, "MFMA patch solve: 3D, K = 8, two patches per block");
@@ -326,13 +407,16 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
                          ls[plc][0], ls[plc][1], ls[plc][DIM > 2 ? 2 : 1], w, t, owner);
   }
   if(!valid) return;
-  const bool keep = !x_is_zero && w != CT(1);
-#pragma unroll
-  for(int a = 0; a < K; ++a)
+  if(!x_is_zero && w != CT(1))
   {
-    VT *p = x + patch_cell<DIM, NP>(v, n, a, ty, tz) * NDC + patch_local<DIM, NP>(a, ty, tz);
-    *p = keep ? VT((CT(1) - w) * CT(*p) + o[a]) : VT(o[a]);
+    CT old[K];
+    load_row<NP>(x + c_lo * NDC + lyz, old);
+    load_row<NP>(x + (c_lo + 1) * NDC + lyz, old + NP);
+#pragma unroll
+    for(int a = 0; a < K; ++a) o[a] += (CT(1) - w) * old[a];
   }
+  store_row<NP>(x + c_lo * NDC + lyz, o);
+  store_row<NP>(x + (c_lo + 1) * NDC + lyz, o + NP);
 }
 
 }  // namespace tpss

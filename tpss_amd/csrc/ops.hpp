@@ -36,6 +36,9 @@ inline unsigned blocks_for(int64_t n, int threads) { return unsigned((n + thread
 
 void operator_apply_any(const tpss_level *lv, void *y, const void *x, const void *b,
                         const uint32_t *cells, int64_t n_list, hipStream_t s);
+// y = A x in FP64 whatever the level's vector precision (setup of fitted local solvers)
+void operator_apply_f64(const tpss_level *lv, double *y, const double *x, const uint32_t *cells,
+                        int64_t n_list, hipStream_t s);
 void fill_zero(void *x, int prec, int64_t n, hipStream_t s);
 void axpby(void *y, const void *x, int prec, int64_t n, double a, double b, hipStream_t s);
 void convert(void *dst, int dp, const void *src, int sp, int64_t n, hipStream_t s);

@@ -19,6 +19,7 @@
 #include "ops.hpp"
 #include "setup_kernels.hpp"
 #include "kernels_mvp.hpp"
+#include "fit_kernels.hpp"
 #include "smoother.hpp"
 
 namespace tpss
@@ -74,9 +75,99 @@ Shape1D<NP> make_shape(int p)
 
 // ---------------------------------------------------------------------------- setup
 
-std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int coloring,
-                                                  hipStream_t s)
+// Separable fit of the exact local matrices (TPSS_LOCAL_FIT): the local
+// matrices of batches of same-red-black-color patches column by column from
+// the FP64 operator, alternating least squares from the surrogate matrices,
+// FD factors of the fit replacing the surrogate ones where the fitted mass
+// matrices are s.p.d.  Requires one factor row per (patch, direction).
+void fit_local_solvers(const tpss_level *lv, LocalSolvers *ls, int sweeps, const double *d_h,
+                       const uint8_t *d_b, const double *d_coef, hipStream_t s)
 {
+  const int dim = ls->dim, K = ls->K, P = ls->P, n = ls->n;
+  const int npd = ipow(K, dim), nprob = P * dim;
+  DeviceArray<double> Mf{size_t(nprob) * K * K}, Lf{size_t(nprob) * K * K};
+  DeviceArray<double> d_err{size_t(P)};
+  dispatch_dim_np(dim, lv->np, [&](auto, auto NPc) {
+    constexpr int NP = decltype(NPc)::value;
+    const auto sh = make_shape<NP>(NP - 1);
+    hipLaunchKernelGGL((k_assemble_1d<NP>), dim3(blocks_for(nprob, 64)), dim3(64), 0, s, d_h, d_b,
+                       d_coef, sh, NP - 1, lv->ip, nprob, Mf.ptr, Lf.ptr);
+    TPSS_HIP(hipGetLastError());
+  });
+  // batches: patches of one red-black color (no shared cells or faces)
+  std::vector<std::vector<int>> by_color(2 << dim);
+  for(int jp = 0; jp < P; ++jp)
+    by_color[patch_color(dim, TPSS_COLOR_REDBLACK, &ls->vtx_unpacked[size_t(jp) * dim])].push_back(jp);
+  const size_t cap_bytes = size_t(4) << 30;
+  const int batch = int(std::max<size_t>(1, std::min<size_t>(
+    size_t(P), cap_bytes / (size_t(npd) * npd * sizeof(double)))));
+  DeviceArray<double> x{size_t(lv->Ns)}, y{size_t(lv->Ns)};
+  fill_zero(x.ptr, TPSS_F64, lv->Ns, s);
+  DeviceArray<double> Abuf{size_t(std::min(batch, P)) * npd * npd};
+  DeviceArray<int> d_plist;
+  DeviceArray<uint32_t> d_cells;
+  for(const auto &members : by_color)
+    for(size_t b0 = 0; b0 < members.size(); b0 += size_t(batch))
+    {
+      const int nb = int(std::min(members.size() - b0, size_t(batch)));
+      std::vector<int> plist(members.begin() + b0, members.begin() + b0 + nb);
+      std::vector<uint32_t> cells;
+      for(int jp : plist)
+        for(int cb = 0; cb < (1 << dim); ++cb)
+        {
+          int64_t c = 0, st = 1;
+          for(int d = 0; d < dim; ++d)
+          {
+            c += int64_t(ls->vtx_unpacked[size_t(jp) * dim + d] - 1 + ((cb >> d) & 1)) * st;
+            st *= n;
+          }
+          cells.push_back(uint32_t(c));
+        }
+      TPSS_HIP(hipStreamSynchronize(s));  // the previous batch is done with the lists
+      d_plist.upload(plist);
+      d_cells.upload(cells);
+      dispatch_dim_np(dim, lv->np, [&](auto D, auto NPc) {
+        constexpr int DIM = decltype(D)::value, KC = 2 * decltype(NPc)::value;
+        for(int col = 0; col < npd; ++col)
+        {
+          hipLaunchKernelGGL((k_fit_unit<DIM, KC>), dim3(blocks_for(nb, 256)), dim3(256), 0, s,
+                             x.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col, 1.0);
+          operator_apply_f64(lv, y.ptr, x.ptr, d_cells.ptr, int64_t(cells.size()), s);
+          hipLaunchKernelGGL((k_fit_gather<DIM, KC>), dim3(blocks_for(int64_t(nb) * npd, 256)),
+                             dim3(256), 0, s, y.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col,
+                             Abuf.ptr);
+          hipLaunchKernelGGL((k_fit_unit<DIM, KC>), dim3(blocks_for(nb, 256)), dim3(256), 0, s,
+                             x.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col, 0.0);
+        }
+        hipLaunchKernelGGL((k_separable_fit<DIM, KC>), dim3(nb), dim3(256), 0, s, Abuf.ptr,
+                           d_plist.ptr, nb, Mf.ptr, Lf.ptr, sweeps, d_err.ptr);
+        TPSS_HIP(hipGetLastError());
+      });
+    }
+  // FD factors of the fits; patches with an indefinite fitted mass keep the surrogate
+  DeviceArray<double> Sf{size_t(nprob) * K * K}, lamf{size_t(nprob) * K};
+  DeviceArray<int> ok{size_t(P)};
+  {
+    std::vector<int> ones(size_t(P), 1);
+    ok.upload(ones);
+  }
+  dispatch_dim_np(dim, lv->np, [&](auto, auto NPc) {
+    constexpr int KC = 2 * decltype(NPc)::value;
+    hipLaunchKernelGGL((k_fit_eig<KC>), dim3(blocks_for(nprob, 64)), dim3(64), 0, s, Mf.ptr, Lf.ptr,
+                       nprob, dim, Sf.ptr, lamf.ptr, ok.ptr);
+    hipLaunchKernelGGL((k_fit_select<KC>), dim3(blocks_for(nprob, 64)), dim3(64), 0, s, Sf.ptr,
+                       lamf.ptr, ok.ptr, nprob, dim, ls->S64.ptr, ls->lam64.ptr, d_err.ptr);
+    TPSS_HIP(hipGetLastError());
+  });
+  TPSS_HIP(hipStreamSynchronize(s));
+  ls->fit_err = d_err.download();
+}
+
+std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv,
+                                                  const tpss_smoother_desc &desc, hipStream_t s)
+{
+  const int coloring = desc.coloring;
+  const bool fit = desc.local_solver == TPSS_LOCAL_FIT;
   auto t0 = std::chrono::steady_clock::now();
   auto ls = std::make_shared<LocalSolvers>();
   const int dim = lv->dim, n = lv->n;
@@ -176,12 +267,24 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int colo
       }
     // deduplicate identical 1D problems; with component blocks (elasticity) the
     // 1D matrix of block c in direction d also depends on whether c == d
+    // (the fitted local solvers need one factor row per patch and direction)
     std::map<std::tuple<double, double, int, int, double, double>, uint32_t> uniq;
     const size_t rows = size_t(P) * dim;
     ls->fac.resize(rows * ls->ncomp);
     for(int c = 0; c < ls->ncomp; ++c)
       for(size_t i = 0; i < rows; ++i)
       {
+        if(fit)
+        {
+          ls->prob_h.push_back(hs[2 * i]);
+          ls->prob_h.push_back(hs[2 * i + 1]);
+          ls->prob_b.push_back(bd[2 * i]);
+          ls->prob_b.push_back(bd[2 * i + 1]);
+          ls->prob_coef.push_back(1.0);
+          ls->prob_coef.push_back(1.0);
+          ls->fac[i] = uint32_t(i);
+          continue;
+        }
         const int d = int(i % dim);
         double a_cons = 1.0, a_pen = 1.0;
         if(ls->ncomp > 1)
@@ -207,7 +310,7 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int colo
         else
           ls->fac[c * rows + i] = it->second;
       }
-    ls->nfac = int(uniq.size());
+    ls->nfac = fit ? int(rows) : int(uniq.size());
   }
   ls->h_host = hs;
   ls->d_fac.upload(ls->fac);
@@ -241,6 +344,21 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv, int colo
     TPSS_HIP(hipStreamSynchronize(s));
     if(status[0]) throw Error(TPSS_E_SINGULAR, "local solver setup: mass matrix not s.p.d.");
     if(status[1]) throw Error(TPSS_E_SINGULAR, "local solver setup: has zero eigenvalues");
+    if(fit)
+    {
+      fit_local_solvers(lv, ls.get(), desc.fit_sweeps > 0 ? desc.fit_sweeps : 4, d_h.ptr, d_b.ptr,
+                        d_coef.ptr, s);
+      TPSS_HIP(hipMemsetAsync(d_status.ptr + 1, 0, sizeof(int), s));
+      dispatch_dim_np(dim, lv->np, [&](auto D, auto NPc) {
+        constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
+        hipLaunchKernelGGL((k_check_eigenvalue_sums<DIM, 2 * NP>), dim3(blocks_for(nrows, 64)),
+                           dim3(64), 0, s, ls->d_fac.ptr, ls->lam64.ptr, nrows, d_status.ptr + 1);
+        TPSS_HIP(hipGetLastError());
+      });
+      TPSS_HIP(hipMemcpyAsync(status, d_status.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+      TPSS_HIP(hipStreamSynchronize(s));
+      if(status[1]) throw Error(TPSS_E_SINGULAR, "local solver setup: fitted solver has zero eigenvalues");
+    }
   }
   ls->S32.resize(ls->S64.n);
   ls->lam32.resize(ls->lam64.n);
@@ -537,10 +655,60 @@ void precond_vmult_any(tpss_smoother *sm, void *dst, const void *src, hipStream_
     precond_dispatch<float>(sm, (float *)dst, (const float *)src, s, accumulate, alpha);
 }
 
-// SmootherRichardson::step_impl: x <- x + omega_R P (b - A x), n_steps times
-void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s)
+// True if a Richardson step with omega_R = 1 can run as residual-free color
+// passes on (x, b) directly: for the multiplicative variant
+//   x + P (b - A x)  ==  the color sweep started from x,
+// because each patch update x_j <- (1 - w) x_j + w A_j^{-1} (b_j - B_j x_out)
+// equals y_j <- y_j + w A_j^{-1} (r - A y)_j on y = x - x_0, r = b - A x_0.
+bool fused_step_applies(const tpss_smoother *sm)
 {
   const tpss_level *lv = sm->level;
+  const LocalSolvers &ls = *sm->ls;
+  if(sm->desc.variant != TPSS_MULTIPLICATIVE || lv->distorted || lv->ncomp != 1 ||
+     ls.coloring != TPSS_COLOR_REDBLACK || !sm->fused_mvp || sm->desc.damping_factor != 1.0)
+    return false;
+  // BF16 patch solves stay on the correction (precond on the residual): in the
+  // residual-free step they would produce the solution values themselves and
+  // cap the smoother's accuracy at BF16 resolution
+  if(sm->desc.local_precision == TPSS_BF16) return false;
+  return !(lv->vec_prec == TPSS_F32 && sm->desc.local_precision == TPSS_F64);
+}
+
+template<typename VT>
+void fused_sweep(tpss_smoother *sm, VT *x, const VT *b, bool x_is_zero, hipStream_t s)
+{
+  const LocalSolvers &ls = *sm->ls;
+  const int lp = sm->desc.local_precision;
+  for(size_t k = 0; k < sm->seq.size(); ++k)
+  {
+    const int c = sm->seq[k];
+    const bool zero = x_is_zero && k == 0;
+    SectionTimer t(sm, 1, s);
+    if(lp == TPSS_F64)
+      mvp_fused_pass<VT, double>(sm, b, x, ls.color_ptr[c], ls.color_ptr[c + 1], zero, s);
+    else
+      mvp_fused_pass<VT, float>(sm, b, x, ls.color_ptr[c], ls.color_ptr[c + 1], zero, s);
+  }
+}
+
+// SmootherRichardson::step_impl: x <- x + omega_R P (b - A x), n_steps times.
+// x_is_zero: the caller guarantees dst == 0 (SmootherRichardson::vmult).
+void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
+                       bool x_is_zero)
+{
+  const tpss_level *lv = sm->level;
+  if(fused_step_applies(sm))
+  {
+    SectionTimer step_timer(sm, 4, s);
+    for(int k = 0; k < sm->desc.n_steps; ++k)
+    {
+      if(lv->vec_prec == TPSS_F64)
+        fused_sweep<double>(sm, (double *)dst, (const double *)src, x_is_zero && k == 0, s);
+      else
+        fused_sweep<float>(sm, (float *)dst, (const float *)src, x_is_zero && k == 0, s);
+    }
+    return;
+  }
   const size_t bytes = size_t(lv->N) * elem_size(lv->vec_prec);
   sm->rich_r.resize(bytes);
   for(int k = 0; k < sm->desc.n_steps; ++k)
@@ -591,6 +759,9 @@ void validate_desc(const tpss_smoother_desc *d)
   TPSS_REQUIRE(d->n_steps >= 1, "n_steps must be >= 1");
   TPSS_REQUIRE(d->local_precision >= TPSS_F64 && d->local_precision <= TPSS_BF16,
                "invalid local precision");
+  TPSS_REQUIRE(d->local_solver == TPSS_LOCAL_SURROGATE || d->local_solver == TPSS_LOCAL_FIT,
+               "invalid local solver");
+  TPSS_REQUIRE(d->fit_sweeps >= 0 && d->fit_sweeps <= 100, "fit_sweeps must be in 0..100");
   TPSS_REQUIRE(!(d->variant == TPSS_MULTIPLICATIVE && d->coloring == TPSS_COLOR_NONE),
                "the multiplicative variant needs a coloring (RedBlack for DG)");
 }
@@ -610,10 +781,12 @@ int tpss_smoother_create(tpss_level *level, const tpss_smoother_desc *desc, tpss
       throw Error(TPSS_E_UNSUPPORTED, "local precision F64 needs F64 level vectors");
     if(level->ncomp > 1 && desc->local_precision == TPSS_BF16)
       throw Error(TPSS_E_UNSUPPORTED, "linear elasticity: local precision F64 or F32");
+    if(level->ncomp > 1 && desc->local_solver == TPSS_LOCAL_FIT)
+      throw Error(TPSS_E_UNSUPPORTED, "fitted local solvers: Poisson levels only");
     auto sm = std::make_unique<tpss_smoother>();
     sm->level = level;
     sm->desc = *desc;
-    sm->ls = build_local_solvers(level, desc->coloring, nullptr);
+    sm->ls = build_local_solvers(level, *desc, nullptr);
     sm->seq = color_sequence(sm->ls->n_colors, desc->reverse != 0, desc->symmetrized != 0);
     sm->timers[2].time = sm->ls->setup_seconds;
     sm->timers[2].n_calls = 1;
@@ -678,7 +851,7 @@ int tpss_smoother_vmult(tpss_smoother *sm, void *dst, const void *src, void *str
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
     fill_zero(dst, sm->level->vec_prec, sm->level->N, as_stream(stream));
-    smoother_step_any(sm, dst, src, as_stream(stream));
+    smoother_step_any(sm, dst, src, as_stream(stream), true);
     check_finite(sm, dst, as_stream(stream));
   });
 }
@@ -742,6 +915,15 @@ int tpss_smoother_patch_h(const tpss_smoother *sm, double *out)
   return guarded([&] {
     TPSS_REQUIRE(sm && out, "null argument");
     std::copy(sm->ls->h_host.begin(), sm->ls->h_host.end(), out);
+  });
+}
+
+int tpss_smoother_fit_errors(const tpss_smoother *sm, double *out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && out, "null argument");
+    TPSS_REQUIRE(!sm->ls->fit_err.empty(), "the smoother has no fitted local solvers");
+    std::copy(sm->ls->fit_err.begin(), sm->ls->fit_err.end(), out);
   });
 }
 

@@ -30,6 +30,7 @@ struct LocalSolvers
   DeviceArray<double> S64, lam64;  // nfac * K * K, nfac * K
   DeviceArray<float> S32, lam32;
   std::vector<double> h_host;      // P * dim * 2 surrogate lengths
+  std::vector<double> fit_err;     // P: relative error of the separable fit (-1: surrogate kept)
   std::vector<std::shared_ptr<DeviceArray<uint32_t>>> color_cells;
   std::vector<int64_t> color_ncells;
   double setup_seconds = 0.0;
@@ -66,7 +67,8 @@ namespace tpss
 {
 void precond_vmult_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
                        bool accumulate, double alpha);
-void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s);
+void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
+                       bool x_is_zero = false);
 template<typename VT>
 void precond_vmult_bf16(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
                         double alpha);
