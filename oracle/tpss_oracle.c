@@ -832,9 +832,17 @@ int or_setup(or_ctx *c)
       double S[3 * OR_MAXK * OR_MAXK], lam[3 * OR_MAXK];
       int bad = 0;
       for(int d = 0; d < dim; ++d) bad |= or_gen_eig(k, L + d * k * k, M + d * k * k, S + d * k * k, lam + d * k);
+      if(!bad)
+      {
+        /* the fit must be s.p.d.: smallest Kronecker-sum eigenvalue positive */
+        double lo = 0.0, hi = 0.0;
+        for(int d = 0; d < dim; ++d) { lo += lam[d * k]; hi += fabs(lam[d * k + k - 1]); }
+        bad = !(lo > 1e-12 * hi);
+      }
       if(bad)
       {
-        ++c->n_fit_fallback; /* fitted mass not s.p.d.: keep the surrogate factors */
+        ++c->n_fit_fallback; /* keep the surrogate factors */
+        c->fit_err[j] = -1.0;
         continue;
       }
       memcpy(c->S + (long)j * dim * k * k, S, sizeof(double) * dim * k * k);

@@ -337,3 +337,20 @@ def test_residual_free_richardson_step(dim, p, n, rev, n_steps):
     sm.set_fused_residual(True)
     sm.vmult(d, dev(b))  # from zero
     assert rel_err(host(d), ref.smoother_vmult(b)) < 1e-12
+
+
+def test_gmg_iterative_coarse_solve():
+    """Coarse level above the dense-inverse size (3D Q2 on 5^3 coarse cells,
+    3375 DoFs): the coarse CG runs inside the outer CG's preconditioner."""
+    import tpss_amd as t
+    g = t.GMGSolver(3, 2, 5, 2, variant=t.MULTIPLICATIVE, coloring=t.COLOR_REDBLACK)
+    ref = o.GMG(3, 2, 5, 2, variant=o.MULTIPLICATIVE, coloring=o.COLOR_REDBLACK)
+    b = o.splitmix_uniform(2, g.n_dofs)
+    z = dev(np.zeros(g.n_dofs))
+    g.vcycle(z, dev(b))
+    assert rel_err(host(z), ref.vcycle(b)) < 1e-9
+    x = dev(np.zeros(g.n_dofs))
+    info, hist = g.solve(x, dev(b), rel_tol=1e-8)
+    xr, itr, histr = ref.cg(b, rel_tol=1e-8)
+    assert info.iterations == itr
+    assert rel_err(host(x), xr) < 1e-7

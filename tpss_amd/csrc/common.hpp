@@ -184,8 +184,5 @@ struct tpss_level
   bool distorted = false;
   std::vector<double> vertices;            // host copy ((n+1)^dim * dim)
   tpss::DeviceArray<double> d_vertices;
-  // distorted-mesh operator: per-cell geometry tables (FP64 + FP32 copies)
-  tpss::DeviceArray<double> d_geo64;
-  tpss::DeviceArray<float> d_geo32;
   tpss::Operator1D op1d;
 };

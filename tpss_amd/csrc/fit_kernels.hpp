@@ -299,6 +299,25 @@ k_fit_eig(const double *__restrict__ Mg, const double *__restrict__ Lg, const in
   for(int e = 0; e < K; ++e) lam_out[size_t(i) * K + e] = lam[e];
 }
 
+// A fit is kept only if it is s.p.d.: every direction's mass is s.p.d. (checked
+// by k_fit_eig) and the smallest eigenvalue of the Kronecker sum,
+// sum_d lambda_d,min, is positive relative to the largest.  One thread per patch.
+template<int K>
+__global__ void __launch_bounds__(64)
+k_fit_validate(const double *__restrict__ lam_fit, const int P, const int dim,
+               int *__restrict__ ok)
+{
+  const int pj = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(pj >= P || !ok[pj]) return;
+  double lo = 0.0, hi = 0.0;
+  for(int d = 0; d < dim; ++d)
+  {
+    lo += lam_fit[(size_t(pj) * dim + d) * K];
+    hi += fabs(lam_fit[(size_t(pj) * dim + d) * K + K - 1]);
+  }
+  if(!(lo > 1e-12 * hi)) ok[pj] = 0;
+}
+
 // factors of patches whose fit succeeded replace the surrogate ones
 template<int K>
 __global__ void __launch_bounds__(64)

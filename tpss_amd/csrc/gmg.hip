@@ -91,14 +91,16 @@ double read_scalar(const double *dptr, hipStream_t s)
 }
 
 // Preconditioned CG in FP64 (deal.II SolverCG semantics, x0 = 0,
-// stop when ||r_k|| <= tol ||r_0||).  Scalars stay on the device; one
-// host synchronisation per iteration for the convergence test.
+// stop when ||r_k|| <= tol ||r_0||).  Scalars stay on the device (sc: 5
+// values of this solve's own, the coarse CG runs inside the outer CG's
+// preconditioner); one host synchronisation per iteration for the
+// convergence test.
 int pcg(tpss_gmg *g, tpss_level *A, double *x, const double *b, double tol, int maxit,
         const std::function<void(double *, const double *)> &precond, hipStream_t s,
-        std::vector<double> *hist, double *r, double *z, double *p, double *Ap)
+        std::vector<double> *hist, double *r, double *z, double *p, double *Ap, double *sc)
 {
   const int64_t N = A->N;
-  double *sc = g->scal.ptr;  // [rz, pAp, rr, rz_new, rz_old]
+  // sc: [rz, pAp, rr, rz_new, rz_old]
   fill_zero(x, TPSS_F64, N, s);
   TPSS_HIP(hipMemcpyAsync(r, b, N * sizeof(double), hipMemcpyDeviceToDevice, s));
   dot64(g, r, r, N, sc + 2, s);
@@ -220,7 +222,7 @@ void coarse_solve(tpss_gmg *g, double *x, const double *b, hipStream_t s)
     return;
   }
   const int it = pcg(g, g->coarse64, x, b, g->desc.coarse_tol, 100000, nullptr, s, nullptr,
-                     g->cr.ptr, g->cz.ptr, g->cp.ptr, g->cAp.ptr);
+                     g->cr.ptr, g->cz.ptr, g->cp.ptr, g->cAp.ptr, g->scal.ptr + 8);
   if(it < 0) throw Error(TPSS_E_NOT_CONVERGED, "coarse CG did not converge");
 }
 
@@ -337,7 +339,7 @@ int tpss_gmg_create(const tpss_gmg_desc *desc, tpss_gmg **out)
       g->z.resize(N);
       g->p.resize(N);
       g->Ap.resize(N);
-      g->scal.resize(8);
+      g->scal.resize(16);  // outer CG [0, 5), coarse CG [8, 13)
     }
     catch(...)
     {
@@ -397,7 +399,7 @@ int tpss_gmg_solve(tpss_gmg *g, void *x, const void *b, double rel_tol, int max_
     std::vector<double> hist;
     auto pre = [&](double *z, const double *r) { apply_mg(g, z, r, s); };
     const int it = pcg(g, g->fine64, (double *)x, (const double *)b, rel_tol, max_iter, pre, s,
-                       &hist, g->r.ptr, g->z.ptr, g->p.ptr, g->Ap.ptr);
+                       &hist, g->r.ptr, g->z.ptr, g->p.ptr, g->Ap.ptr, g->scal.ptr);
     TPSS_HIP(hipStreamSynchronize(s));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if(residual_history)

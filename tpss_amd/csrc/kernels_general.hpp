@@ -118,7 +118,7 @@ __device__ __forceinline__ double contract(const double *src, const double *M, c
 }
 
 template<int DIM, int NP, typename VT, bool RESIDUAL>
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(128, 2)
 k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
                const double *__restrict__ vert, const GeneralTab<NP> tab, const int n,
                const double ip, const int p, const uint32_t *__restrict__ cells)
@@ -129,9 +129,14 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
   __shared__ double F[DIM][NDC];
   __shared__ double X[8][3], XN[8][3];
   __shared__ double FV[NF], FH[DIM][NF];
+  // traces contracted along the face normal: value (PA) and normal derivative
+  // (PB) of the own cell [0] and the neighbour [1] on the NF tangential nodes
+  __shared__ double PA[2][NF], PB[2][NF];
+  __shared__ GeneralTab<NP> tl;  // the 1D tables, indexed at run time
 
   const int64_t cell = cells ? int64_t(cells[blockIdx.x]) : int64_t(blockIdx.x);
   const int t = threadIdx.x;
+  if(t == 0) tl = tab;
   const bool act = t < NDC;
   const int ts = act ? t : 0;
   int cc[3] = {0, 0, 0};
@@ -166,7 +171,7 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
 #pragma unroll
     for(int e = 0; e < DIM; ++e)
     {
-      const double v = contract<DIM, NP, false>(src, e == a ? tab.Dv : tab.V, tid, ts, e);
+      const double v = contract<DIM, NP, false>(src, e == a ? tl.Dv : tl.V, tid, ts, e);
       double *dst = (e % 2 == 0) ? A : B;
       __syncthreads();
       if(act) dst[t] = v;
@@ -181,8 +186,8 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
     double w = 1.0;
     for(int e = 0; e < DIM; ++e)
     {
-      xi[e] = tab.qx[tid[e]];
-      w *= tab.qw[tid[e]];
+      xi[e] = tl.qx[tid[e]];
+      w *= tl.qw[tid[e]];
     }
     map_jacobian<DIM>(X, xi, J);
     const double det = invert<DIM>(J, Ji);
@@ -211,7 +216,7 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
 #pragma unroll
     for(int e = 0; e < DIM; ++e)
     {
-      const double v = contract<DIM, NP, true>(src, e == a ? tab.Dv : tab.V, tid, ts, e);
+      const double v = contract<DIM, NP, true>(src, e == a ? tl.Dv : tl.V, tid, ts, e);
       double *dst = (e % 2 == 0) ? A : B;
       __syncthreads();
       if(act) dst[t] = v;
@@ -225,6 +230,7 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
   // ---------------- face terms
   const double gamma = p == 0 ? 1.0 : double(p) * (p + 1);
   int64_t cstride = 1;
+#pragma unroll 1
   for(int d = 0; d < DIM; ++d)
   {
     int e1 = -1, e2 = -1;
@@ -236,6 +242,7 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
         else
           e2 = e;
       }
+#pragma unroll 1
     for(int side = 0; side < 2; ++side)
     {
       const bool interior = side == 1 ? cc[d] < n - 1 : cc[d] > 0;
@@ -249,46 +256,62 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
         load_corners<DIM>(vert, n, cn, XN, t);
       }
       __syncthreads();
+      // contraction along the normal direction, all tangential nodes at once
+      if(t < 2 * NF)
+      {
+        const int sd = t / NF, m = t % NF;
+        if(sd == 0 || interior)
+        {
+          const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+          constexpr int ST[3] = {1, NP, NP * NP};
+          const int base = i1 * ST[e1] + (DIM == 3 ? i2 * ST[e2 < 0 ? 0 : e2] : 0);
+          const double *src = sd == 0 ? U : UN;
+          const int fs = sd == 0 ? side : nside;
+          double a = 0.0, bsum = 0.0;
+#pragma unroll
+          for(int id = 0; id < NP; ++id)
+          {
+            const double u = src[base + id * ST[d]];
+            a += tl.f[fs][id] * u;
+            bsum += tl.fd[fs][id] * u;
+          }
+          PA[sd][m] = a;
+          PB[sd][m] = bsum;
+        }
+      }
+      __syncthreads();
       if(t < NF)
       {
         const int l1 = t % NP, l2 = DIM == 3 ? t / NP : 0;
-        double xio[3], xin[3], wq = tab.qw[l1] * (DIM == 3 ? tab.qw[l2] : 1.0);
+        double xio[3], xin[3], wq = tl.qw[l1] * (DIM == 3 ? tl.qw[l2] : 1.0);
         xio[d] = double(side);
         xin[d] = double(nside);
-        xio[e1] = xin[e1] = tab.qx[l1];
-        if(DIM == 3) xio[e2] = xin[e2] = tab.qx[l2];
-        // trace values and reference gradients (own: g_o, neighbour: g_n)
+        xio[e1] = xin[e1] = tl.qx[l1];
+        if(DIM == 3) xio[e2] = xin[e2] = tl.qx[l2];
+        // trace values and reference gradients (own: g_o, neighbour: g_n) by
+        // tangential interpolation of the normal contractions
         double uo = 0, un = 0, go[3] = {0, 0, 0}, gn[3] = {0, 0, 0};
-        constexpr int ST[3] = {1, NP, NP * NP};
-        for(int k = 0; k < NDC; ++k)
+        for(int m = 0; m < NF; ++m)
         {
-          int kk[3] = {0, 0, 0};
-          int r = k;
-          for(int e = 0; e < DIM; ++e)
+          const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+          const double t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
+          const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
+          const double t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : 0.0;
+          const double wvv = t1v * t2v, wdv = t1d * t2v, wvd = t1v * t2d;
           {
-            kk[e] = r % NP;
-            r /= NP;
-          }
-          (void)ST;
-          const double t1v = tab.V[l1 * NP + kk[e1]], t1d = tab.Dv[l1 * NP + kk[e1]];
-          const double t2v = DIM == 3 ? tab.V[l2 * NP + kk[e2]] : 1.0;
-          const double t2d = DIM == 3 ? tab.Dv[l2 * NP + kk[e2]] : 0.0;
-          {
-            const double fv = tab.f[side][kk[d]], fdv = tab.fd[side][kk[d]];
-            const double val = U[k];
-            uo += fv * t1v * t2v * val;
-            go[d] += fdv * t1v * t2v * val;
-            go[e1] += fv * t1d * t2v * val;
-            if(DIM == 3) go[e2] += fv * t1v * t2d * val;
+            const double a = PA[0][m];
+            uo += wvv * a;
+            go[d] += wvv * PB[0][m];
+            go[e1] += wdv * a;
+            if(DIM == 3) go[e2 < 0 ? 0 : e2] += wvd * a;
           }
           if(interior)
           {
-            const double fv = tab.f[nside][kk[d]], fdv = tab.fd[nside][kk[d]];
-            const double val = UN[k];
-            un += fv * t1v * t2v * val;
-            gn[d] += fdv * t1v * t2v * val;
-            gn[e1] += fv * t1d * t2v * val;
-            if(DIM == 3) gn[e2] += fv * t1v * t2d * val;
+            const double a = PA[1][m];
+            un += wvv * a;
+            gn[d] += wvv * PB[1][m];
+            gn[e1] += wdv * a;
+            if(DIM == 3) gn[e2 < 0 ? 0 : e2] += wvd * a;
           }
         }
         double J[3][3], Jio[3][3], Jin[3][3];
@@ -330,7 +353,7 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
         double ho, hn = 1.0;
         {
           double xi0[3], J0[3][3], Ji0[3][3], n0[3], nn0 = 0.0;
-          for(int e = 0; e < DIM; ++e) xi0[e] = e == d ? double(side) : tab.qx[0];
+          for(int e = 0; e < DIM; ++e) xi0[e] = e == d ? double(side) : tl.qx[0];
           map_jacobian<DIM>(X, xi0, J0);
           invert<DIM>(J0, Ji0);
           for(int a = 0; a < DIM; ++a)
@@ -378,14 +401,14 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
       {
         // test functions of this cell: phi_i = f(i_d) * tangential
         const int id = tid[d], i1 = tid[e1], i2 = DIM == 3 ? tid[e2] : 0;
-        const double fv = tab.f[side][id], fdv = tab.fd[side][id];
+        const double fv = tl.f[side][id], fdv = tl.fd[side][id];
         double s = 0.0;
         for(int l = 0; l < NF; ++l)
         {
           const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
-          const double t1v = tab.V[l1 * NP + i1], t1d = tab.Dv[l1 * NP + i1];
-          const double t2v = DIM == 3 ? tab.V[l2 * NP + i2] : 1.0;
-          const double t2d = DIM == 3 ? tab.Dv[l2 * NP + i2] : 0.0;
+          const double t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
+          const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
+          const double t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : 0.0;
           s += fv * t1v * t2v * FV[l];
           s += fdv * t1v * t2v * FH[d][l];
           s += fv * t1d * t2v * FH[e1][l];

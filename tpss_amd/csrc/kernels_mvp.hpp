@@ -159,8 +159,11 @@ __device__ __forceinline__ void fd_mfma_pair(float (*B2)[512], const float (*S)[
 
 // FD = 0: vector FMA contractions in CT; FD = 1: BF16 MFMA contractions with
 // FP32 accumulation (3D, K = 8, CT = float)
+// FP32 patch solves: 5 waves per SIMD (<= 96 VGPRs) so that more patches keep
+// their loads in flight while others compute
 template<int DIM, int K, typename VT, typename CT, int FD = 0>
-__global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS))
+__global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS),
+                                  (std::is_same<CT, float>::value && FD == 0 ? 5 : 1))
 k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, K / 2> tab,
                   const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
                   const CT *__restrict__ S_tab, const CT *__restrict__ lam_tab,

@@ -78,8 +78,9 @@ Shape1D<NP> make_shape(int p)
 // Separable fit of the exact local matrices (TPSS_LOCAL_FIT): the local
 // matrices of batches of same-red-black-color patches column by column from
 // the FP64 operator, alternating least squares from the surrogate matrices,
-// FD factors of the fit replacing the surrogate ones where the fitted mass
-// matrices are s.p.d.  Requires one factor row per (patch, direction).
+// FD factors of the fit replacing the surrogate ones where the fit is s.p.d.
+// (fitted masses s.p.d., smallest Kronecker-sum eigenvalue positive).  Requires
+// one factor row per (patch, direction).
 void fit_local_solvers(const tpss_level *lv, LocalSolvers *ls, int sweeps, const double *d_h,
                        const uint8_t *d_b, const double *d_coef, hipStream_t s)
 {
@@ -155,6 +156,8 @@ void fit_local_solvers(const tpss_level *lv, LocalSolvers *ls, int sweeps, const
     constexpr int KC = 2 * decltype(NPc)::value;
     hipLaunchKernelGGL((k_fit_eig<KC>), dim3(blocks_for(nprob, 64)), dim3(64), 0, s, Mf.ptr, Lf.ptr,
                        nprob, dim, Sf.ptr, lamf.ptr, ok.ptr);
+    hipLaunchKernelGGL((k_fit_validate<KC>), dim3(blocks_for(P, 64)), dim3(64), 0, s, lamf.ptr, P,
+                       dim, ok.ptr);
     hipLaunchKernelGGL((k_fit_select<KC>), dim3(blocks_for(nprob, 64)), dim3(64), 0, s, Sf.ptr,
                        lamf.ptr, ok.ptr, nprob, dim, ls->S64.ptr, ls->lam64.ptr, d_err.ptr);
     TPSS_HIP(hipGetLastError());

@@ -21,9 +21,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "damping_dgq3_3d.npy")
 
 
-def distorted_vertices(dim, n, amplitude, seed):
+def distorted_vertices(dim, n, amplitude, seed, graded=False):
     """Interior vertices moved by (amplitude/n) * U[-1,1]^dim; U from SplitMix64
-    keyed by (seed, vertex*dim + d); boundary vertices fixed."""
+    keyed by (seed, vertex*dim + d); boundary vertices fixed.  graded=True scales
+    the amplitude by the vertex's x coordinate (no distortion at x = 0, the full
+    amplitude at x = 1)."""
     g = np.linspace(0.0, 1.0, n + 1)
     mesh = np.meshgrid(*([g] * dim), indexing="ij")
     v = np.stack([m.transpose(tuple(range(dim))[::-1]).ravel() for m in mesh], axis=-1)
@@ -42,7 +44,8 @@ def distorted_vertices(dim, n, amplitude, seed):
         c = t % (n + 1)
         interior &= (c > 0) & (c < n)
         t //= n + 1
-    v[interior] += (amplitude / n) * noise[interior]
+    amp = amplitude * (v[:, :1] if graded else 1.0)
+    v[interior] += ((amp / n) * noise)[interior]
     return v
 
 
@@ -58,10 +61,17 @@ def patch_samples(dim, degree, n, amplitude, seed, ip_factor=4.0, n_samples=4,
     return X, y
 
 
-def generate(dim=3, degree=3, n=32, amplitude=0.2, seeds=range(101, 109), n_samples=4):
+# up to 0.2 h: at 0.25 h and more the perturbed trilinear cells of a 32^3 mesh
+# already have corners with negative Jacobian determinant
+TRAIN_AMPLITUDES = (0.05, 0.08, 0.1, 0.12, 0.15, 0.18, 0.2)
+
+
+def generate(dim=3, degree=3, n=32, amplitudes=TRAIN_AMPLITUDES, seed0=101, n_samples=4):
+    """Training set: one distorted mesh per amplitude (seeds seed0, seed0 + 1, ...),
+    so that the features span mild to strong distortion."""
     Xs, ys = [], []
-    for s in seeds:
-        X, y = patch_samples(dim, degree, n, amplitude, s, n_samples=n_samples)
+    for i, a in enumerate(amplitudes):
+        X, y = patch_samples(dim, degree, n, a, seed0 + i, n_samples=n_samples)
         Xs.append(X)
         ys.append(y)
     return np.concatenate(Xs), np.concatenate(ys)
@@ -90,14 +100,14 @@ def main():
     ap.add_argument("--out", default=DEFAULT_MODEL)
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--degree", type=int, default=3)
-    ap.add_argument("--amplitude", type=float, default=0.2)
-    ap.add_argument("--n-meshes", type=int, default=7)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--amplitudes", default=",".join(str(a) for a in TRAIN_AMPLITUDES))
+    ap.add_argument("--epochs", type=int, default=60)
     args = ap.parse_args()
     t0 = time.time()
-    X, y = generate(3, args.degree, args.n, args.amplitude, seeds=range(101, 101 + args.n_meshes))
+    amps = [float(a) for a in args.amplitudes.split(",")]
+    X, y = generate(3, args.degree, args.n, amps)
     t1 = time.time()
-    print(f"training data: {X.shape[0]} patches from {args.n_meshes} meshes in {t1 - t0:.1f} s; "
+    print(f"training data: {X.shape[0]} patches from {len(amps)} meshes in {t1 - t0:.1f} s; "
           f"target mean {y.mean():.4f} std {y.std():.4f} min {y.min():.3f} max {y.max():.3f}",
           flush=True)
     mlp, l0, l1 = train(X, y, epochs=args.epochs)
