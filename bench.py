@@ -25,6 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+# HBM bytes per launch of the FP32 fused color pass on the 64^3 Q3 level, from two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a copy kernel of
+# known size): profiles/mvp_fused_pass_64cubed_q3_hbm_traffic.md
+FUSED_PASS_TRAFFIC_BYTES = {(64, 3): 2.0 * 47022.4 * 1024 + 39057.8 * 1024}
 
 
 def parse():
@@ -279,7 +283,8 @@ def main():
                    "n_dofs": main_v["n_dofs"], "n_patches": main_v["n_patches"],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": k["gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": k["gbs"] / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": k["gbs"] / HBM_PEAK_GBS,
+                     "traffic": FUSED_PASS_TRAFFIC_BYTES.get((n, p)) if dom == "patch_fd" else None},
         "cpu_baseline": cpu,
         "variants": {name: {"patch_dofs_per_s": r["patch_dofs_per_s"],
                             "ms_per_step": r["ms_per_step"],

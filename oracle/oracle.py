@@ -90,6 +90,7 @@ def _load():
                                                        ctypes.c_int, ctypes.c_double]),
         "or_gmg_destroy": (None, [ctypes.c_void_p]),
         "or_gmg_level": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int]),
+        "or_gmg_smoother": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
         "or_gmg_vcycle": (None, [ctypes.c_void_p, _D, _D]),
         "or_gmg_cg": (ctypes.c_int, [ctypes.c_void_p, _D, _D, ctypes.c_double, ctypes.c_int, _D]),
         "or_ksvd": (ctypes.c_int, [ctypes.c_int] * 4 + [_D, ctypes.c_int, _D, _D, _D]),
@@ -368,6 +369,13 @@ class GMG:
         if h:
             lib.or_gmg_destroy(h)
             self.h = None
+
+    def set_patch_damping(self, level, omega):
+        """per-patch damping of the pre- and post-smoother of one level (oracle patch order)"""
+        self._om = getattr(self, "_om", {})
+        self._om[level] = np.ascontiguousarray(omega, dtype=np.float64)
+        for post in (0, 1):
+            lib.or_set_patch_damping(lib.or_gmg_smoother(self.h, level, post), dp(self._om[level]))
 
     def vcycle(self, b):
         b = np.ascontiguousarray(b, dtype=np.float64)

@@ -1901,6 +1901,7 @@ void or_gmg_destroy(or_gmg *g)
 }
 
 or_ctx *or_gmg_level(or_gmg *g, int level) { return g->pre[level]; }
+or_ctx *or_gmg_smoother(or_gmg *g, int level, int post) { return post ? g->post[level] : g->pre[level]; }
 
 static double dot(long n, const double *a, const double *b)
 {

@@ -86,3 +86,25 @@ def test_training_reduces_loss_and_sets_damping():
     out = d.numpy()
     expect = ref.precond_vmult(src)
     assert np.abs(out - expect).max() < 1e-12 * np.abs(expect).max()
+
+
+def test_shipped_model_beats_fixed_damping():
+    """The trained network (tpss_amd/models/damping_dgq3_3d.npy) against the
+    reference's fixed multiplicative damping 1 (TPSS.h:120-163): GMG-CG on a
+    distorted 16^3 DG Q3 mesh (interior vertices moved by 0.2 h) contracts
+    faster and needs fewer iterations to 1e-8."""
+    t = _tpss()
+    from tpss_amd import learned_damping as ld
+    mlp = ld.load()
+    V = ld.distorted_vertices(3, 16, 0.2, seed=3)
+    res = {}
+    for use in (False, True):
+        g = t.GMGSolver(3, 3, 2, 4, ip_factor=4.0, vertices=V, variant=t.MULTIPLICATIVE,
+                        relaxation=1.0)
+        if use:
+            ld.apply_to_gmg(g, mlp)
+        b = o.splitmix_uniform(2, g.n_dofs)
+        x = t.DeviceVector.zeros(g.n_dofs)
+        info, hist = g.solve(x, t.DeviceVector.from_numpy(b), rel_tol=1e-8, max_iter=100)
+        res[use] = (info.iterations, info.reduction_rate)
+    assert res[True][1] < 0.8 * res[False][1] and res[True][0] < res[False][0], res

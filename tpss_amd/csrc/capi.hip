@@ -88,13 +88,16 @@ void operator_apply(const tpss_level *lv, VT *y, const VT *x, const VT *b, const
       const auto tab = make_elast_tab<NP>(lv);
       const int64_t count = cells ? n_list : lv->ncell;
       if(count == 0) return;
-      const unsigned threads = NDC <= 64 ? 64u : 128u;
+      (void)NDC;
+      constexpr int CPB = elasticity_cells_per_block<DIM, NP>();
+      const unsigned threads = unsigned(elasticity_threads<DIM, NP>());
+      const unsigned grid = blocks_for(count, CPB);
       if(b)
-        hipLaunchKernelGGL((k_elasticity<DIM, NP, VT, true>), dim3(unsigned(count)), dim3(threads),
-                           0, s, x, y, b, tab, lv->n, lv->Ns, cells);
+        hipLaunchKernelGGL((k_elasticity<DIM, NP, VT, true>), dim3(grid), dim3(threads), 0, s, x,
+                           y, b, tab, lv->n, lv->Ns, cells, count);
       else
-        hipLaunchKernelGGL((k_elasticity<DIM, NP, VT, false>), dim3(unsigned(count)),
-                           dim3(threads), 0, s, x, y, b, tab, lv->n, lv->Ns, cells);
+        hipLaunchKernelGGL((k_elasticity<DIM, NP, VT, false>), dim3(grid), dim3(threads), 0, s, x,
+                           y, b, tab, lv->n, lv->Ns, cells, count);
       TPSS_HIP(hipGetLastError());
     });
     return;

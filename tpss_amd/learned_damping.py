@@ -66,7 +66,7 @@ def patch_samples(dim, degree, n, amplitude, seed, ip_factor=4.0, n_samples=4,
 TRAIN_AMPLITUDES = (0.05, 0.08, 0.1, 0.12, 0.15, 0.18, 0.2)
 
 
-def generate(dim=3, degree=3, n=32, amplitudes=TRAIN_AMPLITUDES, seed0=101, n_samples=4):
+def generate(dim=3, degree=3, n=32, amplitudes=TRAIN_AMPLITUDES, seed0=101, n_samples=16):
     """Training set: one distorted mesh per amplitude (seeds seed0, seed0 + 1, ...),
     so that the features span mild to strong distortion."""
     Xs, ys = [], []
@@ -101,11 +101,13 @@ def main():
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--amplitudes", default=",".join(str(a) for a in TRAIN_AMPLITUDES))
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=80)
+    ap.add_argument("--samples", type=int, default=16,
+                    help="random errors per patch in the optimal-damping target")
     args = ap.parse_args()
     t0 = time.time()
     amps = [float(a) for a in args.amplitudes.split(",")]
-    X, y = generate(3, args.degree, args.n, amps)
+    X, y = generate(3, args.degree, args.n, amps, n_samples=args.samples)
     t1 = time.time()
     print(f"training data: {X.shape[0]} patches from {len(amps)} meshes in {t1 - t0:.1f} s; "
           f"target mean {y.mean():.4f} std {y.std():.4f} min {y.min():.3f} max {y.max():.3f}",
