@@ -125,13 +125,16 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
 {
   constexpr int NDC = cpow(NP, DIM);
   constexpr int NF = cpow(NP, DIM - 1);
-  __shared__ double U[NDC], UN[NDC], A[NDC], B[NDC];
+  constexpr int NFACE = 2 * DIM;
+  __shared__ double U[NDC], A[NDC], B[NDC];
+  __shared__ double UN[NFACE][NDC];  // the face neighbours (face f = 2 d + side)
   __shared__ double F[DIM][NDC];
-  __shared__ double X[8][3], XN[8][3];
-  __shared__ double FV[NF], FH[DIM][NF];
-  // traces contracted along the face normal: value (PA) and normal derivative
+  __shared__ double X[8][3], XN[NFACE][8][3];
+  __shared__ double FV[NFACE][NF], FH[NFACE][DIM][NF];
+  // traces contracted along each face normal: value (PA) and normal derivative
   // (PB) of the own cell [0] and the neighbour [1] on the NF tangential nodes
-  __shared__ double PA[2][NF], PB[2][NF];
+  __shared__ double PA[NFACE][2][NF], PB[NFACE][2][NF];
+  __shared__ double HP[NFACE][2];  // penalty lengths of the own cell and the neighbour
   __shared__ GeneralTab<NP> tl;  // the 1D tables, indexed at run time
 
   const int64_t cell = cells ? int64_t(cells[blockIdx.x]) : int64_t(blockIdx.x);
@@ -227,198 +230,201 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
     __syncthreads();
   }
 
-  // ---------------- face terms
+  // ---------------- face terms, all 2 DIM faces of the cell at once
   const double gamma = p == 0 ? 1.0 : double(p) * (p + 1);
-  int64_t cstride = 1;
-#pragma unroll 1
-  for(int d = 0; d < DIM; ++d)
+  auto face_interior = [&](int f) {
+    const int d = f / 2;
+    return (f % 2) == 1 ? cc[d] < n - 1 : cc[d] > 0;
+  };
   {
-    int e1 = -1, e2 = -1;
-    for(int e = 0; e < DIM; ++e)
-      if(e != d)
-      {
-        if(e1 < 0)
-          e1 = e;
-        else
-          e2 = e;
-      }
-#pragma unroll 1
-    for(int side = 0; side < 2; ++side)
+    int64_t st = 1;
+    for(int f = 0; f < NFACE; ++f)
     {
-      const bool interior = side == 1 ? cc[d] < n - 1 : cc[d] > 0;
-      const int64_t nb = interior ? (side == 1 ? cell + cstride : cell - cstride) : cell;
-      const int nside = 1 - side;
+      const int d = f / 2, side = f % 2;
+      if(d > 0 && side == 0) st *= n;
+      if(!face_interior(f)) continue;
+      const int64_t nb = side == 1 ? cell + st : cell - st;
+      int cn[3] = {cc[0], cc[1], cc[2]};
+      cn[d] += side == 1 ? 1 : -1;
+      if(act) UN[f][t] = double(x[nb * NDC + t]);
+      load_corners<DIM>(vert, n, cn, XN[f], t);
+    }
+  }
+  __syncthreads();
+  constexpr int ST[3] = {1, NP, NP * NP};
+  // contraction along the normal: every face, both sides, all tangential nodes
+  for(int o = t; o < NFACE * 2 * NF; o += blockDim.x)
+  {
+    const int f = o / (2 * NF), sd = (o / NF) % 2, m = o % NF;
+    if(sd == 1 && !face_interior(f)) continue;
+    const int d = f / 2, side = f % 2;
+    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+    const int base = i1 * ST[e1] + (DIM == 3 ? i2 * ST[e2] : 0);
+    const double *src = sd == 0 ? U : UN[f];
+    const int fs = sd == 0 ? side : 1 - side;
+    double a = 0.0, bsum = 0.0;
+#pragma unroll
+    for(int id = 0; id < NP; ++id)
+    {
+      const double u = src[base + id * ST[d]];
+      a += tl.f[fs][id] * u;
+      bsum += tl.fd[fs][id] * u;
+    }
+    PA[f][sd][m] = a;
+    PB[f][sd][m] = bsum;
+  }
+  // penalty lengths h = 1/|(J^{-1} n)_d| at the first face quadrature point of
+  // both cells (apps/laplace_integrator.h:651-656, 692-693), one thread per face
+  if(t < NFACE)
+  {
+    const int f = t, d = f / 2, side = f % 2;
+    double xi0[3], J0[3][3], Ji0[3][3], n0[3], nn0 = 0.0;
+    for(int e = 0; e < DIM; ++e) xi0[e] = e == d ? double(side) : tl.qx[0];
+    map_jacobian<DIM>(X, xi0, J0);
+    invert<DIM>(J0, Ji0);
+    for(int a = 0; a < DIM; ++a)
+    {
+      n0[a] = Ji0[d][a] * (side == 0 ? -1.0 : 1.0);
+      nn0 += n0[a] * n0[a];
+    }
+    nn0 = sqrt(nn0);
+    double sdot = 0.0;
+    for(int a = 0; a < DIM; ++a)
+    {
+      n0[a] /= nn0;
+      sdot += Ji0[d][a] * n0[a];
+    }
+    HP[f][0] = 1.0 / fabs(sdot);
+    HP[f][1] = 1.0;
+    if(face_interior(f))
+    {
+      xi0[d] = double(1 - side);
+      map_jacobian<DIM>(XN[f], xi0, J0);
+      invert<DIM>(J0, Ji0);
+      sdot = 0.0;
+      for(int a = 0; a < DIM; ++a) sdot += Ji0[d][a] * n0[a];
+      HP[f][1] = 1.0 / fabs(sdot);
+    }
+  }
+  __syncthreads();
+  // fluxes at every face quadrature point
+  for(int pt = t; pt < NFACE * NF; pt += blockDim.x)
+  {
+    const int f = pt / NF, l = pt % NF;
+    const int d = f / 2, side = f % 2, nside = 1 - side;
+    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    const bool interior = face_interior(f);
+    const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
+    double xio[3], xin[3];
+    const double wq = tl.qw[l1] * (DIM == 3 ? tl.qw[l2] : 1.0);
+    for(int e = 0; e < DIM; ++e)
+    {
+      const double qe = e == e1 ? tl.qx[l1] : tl.qx[l2];
+      xio[e] = e == d ? double(side) : qe;
+      xin[e] = e == d ? double(nside) : qe;
+    }
+    // traces: value, normal and the two tangential reference derivatives
+    double uo = 0, un = 0, gon = 0, got1 = 0, got2 = 0, gnn = 0, gnt1 = 0, gnt2 = 0;
+    for(int m = 0; m < NF; ++m)
+    {
+      const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+      const double t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
+      const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
+      const double t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : 0.0;
+      const double wvv = t1v * t2v, wdv = t1d * t2v, wvd = t1v * t2d;
+      const double a0 = PA[f][0][m];
+      uo += wvv * a0;
+      gon += wvv * PB[f][0][m];
+      got1 += wdv * a0;
+      got2 += wvd * a0;
       if(interior)
       {
-        int cn[3] = {cc[0], cc[1], cc[2]};
-        cn[d] += side == 1 ? 1 : -1;
-        if(act) UN[t] = double(x[nb * NDC + t]);
-        load_corners<DIM>(vert, n, cn, XN, t);
+        const double a1 = PA[f][1][m];
+        un += wvv * a1;
+        gnn += wvv * PB[f][1][m];
+        gnt1 += wdv * a1;
+        gnt2 += wvd * a1;
       }
-      __syncthreads();
-      // contraction along the normal direction, all tangential nodes at once
-      if(t < 2 * NF)
-      {
-        const int sd = t / NF, m = t % NF;
-        if(sd == 0 || interior)
-        {
-          const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-          constexpr int ST[3] = {1, NP, NP * NP};
-          const int base = i1 * ST[e1] + (DIM == 3 ? i2 * ST[e2 < 0 ? 0 : e2] : 0);
-          const double *src = sd == 0 ? U : UN;
-          const int fs = sd == 0 ? side : nside;
-          double a = 0.0, bsum = 0.0;
-#pragma unroll
-          for(int id = 0; id < NP; ++id)
-          {
-            const double u = src[base + id * ST[d]];
-            a += tl.f[fs][id] * u;
-            bsum += tl.fd[fs][id] * u;
-          }
-          PA[sd][m] = a;
-          PB[sd][m] = bsum;
-        }
-      }
-      __syncthreads();
-      if(t < NF)
-      {
-        const int l1 = t % NP, l2 = DIM == 3 ? t / NP : 0;
-        double xio[3], xin[3], wq = tl.qw[l1] * (DIM == 3 ? tl.qw[l2] : 1.0);
-        xio[d] = double(side);
-        xin[d] = double(nside);
-        xio[e1] = xin[e1] = tl.qx[l1];
-        if(DIM == 3) xio[e2] = xin[e2] = tl.qx[l2];
-        // trace values and reference gradients (own: g_o, neighbour: g_n) by
-        // tangential interpolation of the normal contractions
-        double uo = 0, un = 0, go[3] = {0, 0, 0}, gn[3] = {0, 0, 0};
-        for(int m = 0; m < NF; ++m)
-        {
-          const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-          const double t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
-          const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
-          const double t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : 0.0;
-          const double wvv = t1v * t2v, wdv = t1d * t2v, wvd = t1v * t2d;
-          {
-            const double a = PA[0][m];
-            uo += wvv * a;
-            go[d] += wvv * PB[0][m];
-            go[e1] += wdv * a;
-            if(DIM == 3) go[e2 < 0 ? 0 : e2] += wvd * a;
-          }
-          if(interior)
-          {
-            const double a = PA[1][m];
-            un += wvv * a;
-            gn[d] += wvv * PB[1][m];
-            gn[e1] += wdv * a;
-            if(DIM == 3) gn[e2 < 0 ? 0 : e2] += wvd * a;
-          }
-        }
-        double J[3][3], Jio[3][3], Jin[3][3];
-        map_jacobian<DIM>(X, xio, J);
-        const double deto = invert<DIM>(J, Jio);
-        // outward unit normal of this cell and surface element (Nanson)
-        double nv[3], nn = 0.0;
-        for(int a = 0; a < DIM; ++a)
-        {
-          nv[a] = Jio[d][a] * (side == 0 ? -1.0 : 1.0);
-          nn += nv[a] * nv[a];
-        }
-        nn = sqrt(nn);
-        for(int a = 0; a < DIM; ++a) nv[a] /= nn;
-        const double dS = fabs(deto) * nn * wq;
-        double co[3], cn[3] = {0, 0, 0};
-        for(int bb = 0; bb < DIM; ++bb)
-        {
-          double s = 0.0;
-          for(int a = 0; a < DIM; ++a) s += Jio[bb][a] * nv[a];
-          co[bb] = s;
-        }
-        double dno = 0.0;
-        for(int bb = 0; bb < DIM; ++bb) dno += go[bb] * co[bb];
-        double dnn = 0.0;
-        if(interior)
-        {
-          map_jacobian<DIM>(XN, xin, J);
-          invert<DIM>(J, Jin);
-          for(int bb = 0; bb < DIM; ++bb)
-          {
-            double s = 0.0;
-            for(int a = 0; a < DIM; ++a) s += Jin[bb][a] * nv[a];
-            cn[bb] = s;
-          }
-          for(int bb = 0; bb < DIM; ++bb) dnn += gn[bb] * cn[bb];
-        }
-        // penalty from the first face quadrature point (h = 1/|(J^{-1} n)_d|)
-        double ho, hn = 1.0;
-        {
-          double xi0[3], J0[3][3], Ji0[3][3], n0[3], nn0 = 0.0;
-          for(int e = 0; e < DIM; ++e) xi0[e] = e == d ? double(side) : tl.qx[0];
-          map_jacobian<DIM>(X, xi0, J0);
-          invert<DIM>(J0, Ji0);
-          for(int a = 0; a < DIM; ++a)
-          {
-            n0[a] = Ji0[d][a] * (side == 0 ? -1.0 : 1.0);
-            nn0 += n0[a] * n0[a];
-          }
-          nn0 = sqrt(nn0);
-          double s = 0.0;
-          for(int a = 0; a < DIM; ++a)
-          {
-            n0[a] /= nn0;
-            s += Ji0[d][a] * n0[a];
-          }
-          ho = 1.0 / fabs(s);
-          if(interior)
-          {
-            xi0[d] = double(nside);
-            map_jacobian<DIM>(XN, xi0, J0);
-            invert<DIM>(J0, Ji0);
-            s = 0.0;
-            for(int a = 0; a < DIM; ++a) s += Ji0[d][a] * n0[a];
-            hn = 1.0 / fabs(s);
-          }
-        }
-        double fval, fnd;
-        if(interior)
-        {
-          const double sigma = ip * 0.5 * 2.0 * gamma * (0.5 / ho + 0.5 / hn);
-          const double jump = uo - un, avg = 0.5 * (dno + dnn);
-          fval = dS * (jump * sigma - avg);
-          fnd = dS * (-0.5 * jump);
-        }
-        else
-        {
-          const double sigma = ip * 2.0 * gamma / ho;
-          fval = dS * (uo * sigma - dno);
-          fnd = dS * (-uo);
-        }
-        FV[t] = fval;
-        for(int bb = 0; bb < DIM; ++bb) FH[bb][t] = fnd * co[bb];
-      }
-      __syncthreads();
-      if(act)
-      {
-        // test functions of this cell: phi_i = f(i_d) * tangential
-        const int id = tid[d], i1 = tid[e1], i2 = DIM == 3 ? tid[e2] : 0;
-        const double fv = tl.f[side][id], fdv = tl.fd[side][id];
-        double s = 0.0;
-        for(int l = 0; l < NF; ++l)
-        {
-          const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
-          const double t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
-          const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
-          const double t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : 0.0;
-          s += fv * t1v * t2v * FV[l];
-          s += fdv * t1v * t2v * FH[d][l];
-          s += fv * t1d * t2v * FH[e1][l];
-          if(DIM == 3) s += fv * t1v * t2d * FH[e2][l];
-        }
-        acc += s;
-      }
-      __syncthreads();
     }
-    cstride *= n;
+    double J[3][3], Jio[3][3];
+    map_jacobian<DIM>(X, xio, J);
+    const double deto = invert<DIM>(J, Jio);
+    // outward unit normal of this cell and surface element (Nanson)
+    double nv[3] = {0, 0, 0}, nn = 0.0;
+    for(int a = 0; a < DIM; ++a)
+    {
+      nv[a] = Jio[d][a] * (side == 0 ? -1.0 : 1.0);
+      nn += nv[a] * nv[a];
+    }
+    nn = sqrt(nn);
+    for(int a = 0; a < DIM; ++a) nv[a] /= nn;
+    const double dS = fabs(deto) * nn * wq;
+    // reference-gradient coefficients of the normal derivative: co = J^{-1} n
+    double co[3] = {0, 0, 0};
+    for(int bb = 0; bb < DIM; ++bb)
+    {
+      double s2 = 0.0;
+      for(int a = 0; a < DIM; ++a) s2 += Jio[bb][a] * nv[a];
+      co[bb] = s2;
+    }
+    const double dno = gon * co[d] + got1 * co[e1] + (DIM == 3 ? got2 * co[e2] : 0.0);
+    double dnn = 0.0;
+    if(interior)
+    {
+      double Jn[3][3], Jin[3][3];
+      map_jacobian<DIM>(XN[f], xin, Jn);
+      invert<DIM>(Jn, Jin);
+      double cn[3] = {0, 0, 0};
+      for(int bb = 0; bb < DIM; ++bb)
+      {
+        double s2 = 0.0;
+        for(int a = 0; a < DIM; ++a) s2 += Jin[bb][a] * nv[a];
+        cn[bb] = s2;
+      }
+      dnn = gnn * cn[d] + gnt1 * cn[e1] + (DIM == 3 ? gnt2 * cn[e2] : 0.0);
+    }
+    double fval, fnd;
+    if(interior)
+    {
+      const double sigma = ip * 0.5 * 2.0 * gamma * (0.5 / HP[f][0] + 0.5 / HP[f][1]);
+      const double jump = uo - un, avg = 0.5 * (dno + dnn);
+      fval = dS * (jump * sigma - avg);
+      fnd = dS * (-0.5 * jump);
+    }
+    else
+    {
+      const double sigma = ip * 2.0 * gamma / HP[f][0];
+      fval = dS * (uo * sigma - dno);
+      fnd = dS * (-uo);
+    }
+    FV[f][l] = fval;
+    for(int bb = 0; bb < DIM; ++bb) FH[f][bb][l] = fnd * co[bb];
+  }
+  __syncthreads();
+  if(act)
+  {
+    // test functions of this cell: phi_i = f(i_d) * tangential factors, all faces
+    for(int f = 0; f < NFACE; ++f)
+    {
+      const int d = f / 2, side = f % 2;
+      const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+      const int id = tid[d], i1 = tid[e1], i2 = DIM == 3 ? tid[e2] : 0;
+      const double fv = tl.f[side][id], fdv = tl.fd[side][id];
+      double s = 0.0;
+      for(int l = 0; l < NF; ++l)
+      {
+        const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
+        const double t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
+        const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
+        const double t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : 0.0;
+        s += fv * t1v * t2v * FV[f][l];
+        s += fdv * t1v * t2v * FH[f][d][l];
+        s += fv * t1d * t2v * FH[f][e1][l];
+        if(DIM == 3) s += fv * t1v * t2d * FH[f][e2][l];
+      }
+      acc += s;
+    }
   }
   if(act)
   {
