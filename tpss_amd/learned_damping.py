@@ -61,14 +61,18 @@ def patch_samples(dim, degree, n, amplitude, seed, ip_factor=4.0, n_samples=4,
     return X, y
 
 
-# up to 0.2 h: at 0.25 h and more the perturbed trilinear cells of a 32^3 mesh
-# already have corners with negative Jacobian determinant
-TRAIN_AMPLITUDES = (0.05, 0.08, 0.1, 0.12, 0.15, 0.18, 0.2)
+# Seven 32^3 meshes at 0.2 h (seeds 101..107), four random errors per patch in
+# the target, 40 epochs: the settings the shipped model was trained with.  A
+# range of amplitudes (--amplitudes 0.05,0.08,...,0.2) is accepted, but a model
+# trained on it damped the strongly distorted 64^3 test mesh worse (49 CG
+# iterations against 14).  Amplitudes stop at 0.2 h: from 0.25 h on, perturbed
+# trilinear cells of a 32^3 mesh have corners with negative Jacobian determinant.
+TRAIN_AMPLITUDES = (0.2,) * 7
 
 
-def generate(dim=3, degree=3, n=32, amplitudes=TRAIN_AMPLITUDES, seed0=101, n_samples=16):
-    """Training set: one distorted mesh per amplitude (seeds seed0, seed0 + 1, ...),
-    so that the features span mild to strong distortion."""
+def generate(dim=3, degree=3, n=32, amplitudes=TRAIN_AMPLITUDES, seed0=101, n_samples=4):
+    """Training set: one distorted mesh per entry of amplitudes (seeds seed0,
+    seed0 + 1, ...)."""
     Xs, ys = [], []
     for i, a in enumerate(amplitudes):
         X, y = patch_samples(dim, degree, n, a, seed0 + i, n_samples=n_samples)
@@ -101,8 +105,8 @@ def main():
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--amplitudes", default=",".join(str(a) for a in TRAIN_AMPLITUDES))
-    ap.add_argument("--epochs", type=int, default=80)
-    ap.add_argument("--samples", type=int, default=16,
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--samples", type=int, default=4,
                     help="random errors per patch in the optimal-damping target")
     args = ap.parse_args()
     t0 = time.time()
