@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace tpss
 {
@@ -338,14 +339,26 @@ __global__ void k_patch_dof_indices(const uint32_t *__restrict__ vtx, const int 
   out[g] = uint32_t(patch_cell<DIM, NP>(v, n, i0, i1, i2) * NDC + patch_local<DIM, NP>(i0, i1, i2));
 }
 
+// s * w / lam: correctly rounded division in FP64; in FP32 the hardware
+// reciprocal (1 ulp), which is well inside the FP32 patch solve's rounding
+template<typename CT>
+__device__ __forceinline__ CT eig_scale(CT s, CT w, CT lam)
+{
+  if constexpr(std::is_same<CT, float>::value)
+    return s * w * __builtin_amdgcn_rcpf(lam);
+  else
+    return s * w / lam;
+}
+
 // Fast-diagonalization inverse of one patch held as x-lines: thread t owns the
 // x-line (y, z) = (t % K, t / K); on entry r is that line of the patch
 // right-hand side, on exit o is the same line of
 //   u = (S_2 x S_1 x S_0) diag(w / (lam_0 + lam_1 + lam_2)) (S_2^T x S_1^T x S_0^T) r.
 // The line direction rotates x -> y -> z -> y -> x through the patch's LDS
-// buffer B (K^DIM entries); contains __syncthreads (every thread of the block
-// must call it).
-template<int DIM, int K, typename CT>
+// buffer B (entry (x, y, z) at x + SY y + SZ z: padded strides keep the
+// transposes free of LDS bank conflicts); contains __syncthreads (every thread
+// of the block must call it).
+template<int DIM, int K, typename CT, int SY = K, int SZ = K * K>
 __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, const CT *S1,
                                          const CT *S2, const CT *l0, const CT *l1, const CT *l2,
                                          const CT w, const int t, const bool owner)
@@ -365,18 +378,18 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
   {
 #pragma unroll
     for(int i = 0; i < K; ++i)
-      if(owner) B[i + K * ty] = o[i];
+      if(owner) B[i + SY * ty] = o[i];
     __syncthreads();
     const int x0 = t;
 #pragma unroll
-    for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj];
+    for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + SY * jj];
 #pragma unroll
     for(int jj = 0; jj < K; ++jj)
     {
       CT s = 0;
 #pragma unroll
       for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
-      o[jj] = s * w / (l0[x0] + l1[jj]);
+      o[jj] = eig_scale(s, w, l0[x0] + l1[jj]);
     }
 #pragma unroll
     for(int b = 0; b < K; ++b)
@@ -389,21 +402,21 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
     __syncthreads();
 #pragma unroll
     for(int b = 0; b < K; ++b)
-      if(owner) B[x0 + K * b] = r[b];
+      if(owner) B[x0 + SY * b] = r[b];
     __syncthreads();
 #pragma unroll
-    for(int i = 0; i < K; ++i) r[i] = B[i + K * ty];
+    for(int i = 0; i < K; ++i) r[i] = B[i + SY * ty];
   }
   else
   {
 #pragma unroll
     for(int i = 0; i < K; ++i)
-      if(owner) B[i + K * ty + K * K * tz] = o[i];
+      if(owner) B[i + SY * ty + SZ * tz] = o[i];
     __syncthreads();
     {
       const int x0 = t % K, z0 = t / K;
 #pragma unroll
-      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj + K * K * z0];
+      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + SY * jj + SZ * z0];
 #pragma unroll
       for(int jj = 0; jj < K; ++jj)
       {
@@ -414,13 +427,13 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
       }
 #pragma unroll
       for(int jj = 0; jj < K; ++jj)
-        if(owner) B[x0 + K * jj + K * K * z0] = o[jj];
+        if(owner) B[x0 + SY * jj + SZ * z0] = o[jj];
     }
     __syncthreads();
     {
       const int x0 = t % K, y0 = t / K;
 #pragma unroll
-      for(int l = 0; l < K; ++l) r[l] = B[x0 + K * y0 + K * K * l];
+      for(int l = 0; l < K; ++l) r[l] = B[x0 + SY * y0 + SZ * l];
       const CT lxy = l0[x0] + l1[y0];
 #pragma unroll
       for(int l = 0; l < K; ++l)
@@ -428,7 +441,7 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
         CT s = 0;
 #pragma unroll
         for(int c = 0; c < K; ++c) s += S2[c * K + l] * r[c];
-        o[l] = s * w / (lxy + l2[l]);
+        o[l] = eig_scale(s, w, lxy + l2[l]);
       }
 #pragma unroll
       for(int c = 0; c < K; ++c)
@@ -440,13 +453,13 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
       }
 #pragma unroll
       for(int c = 0; c < K; ++c)
-        if(owner) B[x0 + K * y0 + K * K * c] = r[c];
+        if(owner) B[x0 + SY * y0 + SZ * c] = r[c];
     }
     __syncthreads();
     {
       const int x0 = t % K, z0 = t / K;
 #pragma unroll
-      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + K * jj + K * K * z0];
+      for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + SY * jj + SZ * z0];
 #pragma unroll
       for(int b = 0; b < K; ++b)
       {
@@ -457,11 +470,11 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
       }
 #pragma unroll
       for(int b = 0; b < K; ++b)
-        if(owner) B[x0 + K * b + K * K * z0] = o[b];
+        if(owner) B[x0 + SY * b + SZ * z0] = o[b];
     }
     __syncthreads();
 #pragma unroll
-    for(int i = 0; i < K; ++i) r[i] = B[i + K * ty + K * K * tz];
+    for(int i = 0; i < K; ++i) r[i] = B[i + SY * ty + SZ * tz];
   }
   // backward x
 #pragma unroll

@@ -159,11 +159,19 @@ __device__ __forceinline__ void fd_mfma_pair(float (*B2)[512], const float (*S)[
 
 // FD = 0: vector FMA contractions in CT; FD = 1: BF16 MFMA contractions with
 // FP32 accumulation (3D, K = 8, CT = float)
-// FP32 patch solves: 5 waves per SIMD (<= 96 VGPRs) so that more patches keep
-// their loads in flight while others compute
+// Waves per SIMD the register allocation must allow.  Up to K = 8: FP32 patch
+// solves 5 (<= 96 VGPRs) so that more patches keep their loads in flight while
+// others compute; longer lines (K = 10) hold more values per thread: 3.  FP64
+// solves unconstrained (3D K = 8: 186 VGPRs, 2 waves; forcing 3 spills and
+// measured 20 % slower).
+template<int K, typename CT, int FD>
+constexpr int mvp_min_waves()
+{
+  return FD == 1 || !std::is_same<CT, float>::value ? 1 : K <= 8 ? 5 : 3;
+}
+
 template<int DIM, int K, typename VT, typename CT, int FD = 0>
-__global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS),
-                                  (std::is_same<CT, float>::value && FD == 0 ? 5 : 1))
+__global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS), (mvp_min_waves<K, CT, FD>()))
 k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, K / 2> tab,
                   const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
                   const CT *__restrict__ S_tab, const CT *__restrict__ lam_tab,
@@ -172,9 +180,18 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
 {
   using PS = PatchShape<DIM, K>;
   constexpr int NP = PS::NP, L = PS::L, NPD = PS::NPD, NDC = PS::NDC, PPB = PS::PPB;
-  __shared__ CT buf[PPB][NPD];
-  __shared__ CT A1[PPB][NPD];
-  __shared__ CT A2[PPB][DIM > 2 ? NPD : 1];
+  // LDS strides (y, z) of the patch buffers, padded for 3D K = 8 so that the
+  // line transposes of the solve (FSY, FSZ) and the cross-line reads of the
+  // coupling stage (CSY, CSZ) are free of bank conflicts.  The padded solve
+  // buffer reuses the coupling buffer A2, whose last read precedes a barrier
+  // that every thread passes before the solve writes.
+  constexpr bool PAD = FD == 0 && DIM == 3 && K == 8;
+  constexpr int FSY = K, FSZ = PAD ? 72 : K * K;
+  constexpr int CSY = PAD ? 12 : K, CSZ = PAD ? 96 : K * K;
+  static_assert(!PAD || K * FSZ <= K * CSZ, "solve buffer must fit in the coupling buffer");
+  __shared__ CT buf[PPB][PAD ? 1 : NPD];
+  __shared__ CT A1[PPB][DIM > 2 ? K * CSZ : NPD];
+  __shared__ CT A2[PPB][DIM > 2 ? K * CSZ : 1];
   __shared__ CT Ss[PPB][DIM][K * K];
   __shared__ CT ls[PPB][DIM][K];
 
@@ -185,13 +202,21 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   const int plc = pl < PPB ? pl : 0;
   const bool owner = pl < PPB;
 
-  for(int idx = threadIdx.x; idx < PPB * DIM * K * K; idx += blockDim.x)
-  {
-    const int q = idx / (DIM * K * K), r = idx % (DIM * K * K);
-    const int d = r / (K * K), e = r % (K * K);
-    const int jj = j0 + q;
-    Ss[q][d][e] = jj < patch_end ? S_tab[size_t(fac[size_t(jj) * DIM + d]) * K * K + e] : CT(0);
-  }
+  // FP32 solves with one patch per wave (3D, K = 8): the eigenvector matrices
+  // of the wave's patch are read with wave-uniform addresses straight into
+  // scalar registers (operands of the contraction FMAs) instead of being
+  // staged through LDS, which takes most LDS reads off the pass.  (FP64
+  // matrices would need twice the scalar registers and spill.)
+  constexpr bool S_SCALAR =
+      FD == 0 && std::is_same<CT, float>::value && L == 64 && PS::THREADS == PPB * 64;
+  if constexpr(!S_SCALAR)
+    for(int idx = threadIdx.x; idx < PPB * DIM * K * K; idx += blockDim.x)
+    {
+      const int q = idx / (DIM * K * K), r = idx % (DIM * K * K);
+      const int d = r / (K * K), e = r % (K * K);
+      const int jj = j0 + q;
+      Ss[q][d][e] = jj < patch_end ? S_tab[size_t(fac[size_t(jj) * DIM + d]) * K * K + e] : CT(0);
+    }
   for(int idx = threadIdx.x; idx < PPB * DIM * K; idx += blockDim.x)
   {
     const int q = idx / (DIM * K), r = idx % (DIM * K);
@@ -340,8 +365,8 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
     for(int i = 0; i < K; ++i)
       if(owner)
       {
-        Z[i + K * ty + K * K * tz] = z0[i];
-        if(DIM > 2) T2[i + K * ty + K * K * tz] = t2[i];
+        Z[i + CSY * ty + CSZ * tz] = z0[i];
+        if(DIM > 2) T2[i + CSY * ty + CSZ * tz] = t2[i];
       }
     __syncthreads();
     CT a[K], bb[K];
@@ -353,8 +378,8 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
 #pragma unroll
       for(int jj = 0; jj < NP; ++jj)
       {
-        s += tab.M[ly * NP + jj] * Z[i + K * (y0 + jj) + K * K * tz];
-        if(DIM > 2) s2 += tab.M[ly * NP + jj] * T2[i + K * (y0 + jj) + K * K * tz];
+        s += tab.M[ly * NP + jj] * Z[i + CSY * (y0 + jj) + CSZ * tz];
+        if(DIM > 2) s2 += tab.M[ly * NP + jj] * T2[i + CSY * (y0 + jj) + CSZ * tz];
       }
       a[i] = s;
       bb[i] = s2;
@@ -369,7 +394,7 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
       __syncthreads();
 #pragma unroll
       for(int i = 0; i < K; ++i)
-        if(owner) Z[i + K * ty + K * K * tz] = a[i];
+        if(owner) Z[i + CSY * ty + CSZ * tz] = a[i];
       __syncthreads();
       const int z00 = bz * NP;
 #pragma unroll
@@ -377,7 +402,7 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
       {
         CT s = bb[i];
 #pragma unroll
-        for(int kk = 0; kk < NP; ++kk) s += tab.M[lz * NP + kk] * Z[i + K * ty + K * K * (z00 + kk)];
+        for(int kk = 0; kk < NP; ++kk) s += tab.M[lz * NP + kk] * Z[i + CSY * ty + CSZ * (z00 + kk)];
         g[i] -= s;
       }
     }
@@ -406,8 +431,22 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   {
 #pragma unroll
     for(int i = 0; i < K; ++i) r[i] = g[i];
-    fd_lines<DIM, K, CT>(r, o, buf[plc], Ss[plc][0], Ss[plc][1], Ss[plc][DIM > 2 ? 2 : 1],
-                         ls[plc][0], ls[plc][1], ls[plc][DIM > 2 ? 2 : 1], w, t, owner);
+    if constexpr(S_SCALAR)
+    {
+      // wave index as a wave-uniform value; the last patch of the pass stands
+      // in for a missing one (its results are discarded)
+      const int pw = __builtin_amdgcn_readfirstlane(int(threadIdx.x)) / 64;
+      const int jw = min(j0 + pw, patch_end - 1);
+      const CT *S0 = S_tab + size_t(fac[size_t(jw) * DIM + 0]) * K * K;
+      const CT *S1 = S_tab + size_t(fac[size_t(jw) * DIM + 1]) * K * K;
+      const CT *S2 = S_tab + size_t(fac[size_t(jw) * DIM + 2]) * K * K;
+      fd_lines<DIM, K, CT, FSY, FSZ>(r, o, PAD ? A2[plc] : buf[plc], S0, S1, S2, ls[plc][0],
+                                     ls[plc][1], ls[plc][2], w, t, owner);
+    }
+    else
+      fd_lines<DIM, K, CT, FSY, FSZ>(r, o, PAD ? A2[plc] : buf[plc], Ss[plc][0], Ss[plc][1],
+                                     Ss[plc][DIM > 2 ? 2 : 1], ls[plc][0], ls[plc][1],
+                                     ls[plc][DIM > 2 ? 2 : 1], w, t, owner);
   }
   if(!valid) return;
   if(!x_is_zero && w != CT(1))
