@@ -350,30 +350,102 @@ __device__ __forceinline__ CT eig_scale(CT s, CT w, CT lam)
     return s * w / lam;
 }
 
+// o = S^T r (forward) and r = S o (backward) along one line, S[a * K + i] =
+// component a of eigenvector i.  even_odd: the first K/2 eigenvectors are
+// symmetric, the last K/2 antisymmetric under the reversal a -> K-1-a (the 1D
+// problems of patches whose two cells have equal lengths and no boundary
+// face), so each product takes two (K/2 x K/2) blocks of S on the folded line
+// r_a +- r_{K-1-a}: half the multiply-adds.  (A compile-time choice: a
+// run-time one keeps the line arrays out of registers.)
+template<int K, bool EVEN_ODD, typename CT>
+__device__ __forceinline__ void s_forward(const CT *S, const CT *r, CT *o)
+{
+  constexpr int P = K / 2;
+  if constexpr(EVEN_ODD)
+  {
+    CT rp[P], rm[P];
+#pragma unroll
+    for(int a = 0; a < P; ++a)
+    {
+      rp[a] = r[a] + r[K - 1 - a];
+      rm[a] = r[a] - r[K - 1 - a];
+    }
+#pragma unroll
+    for(int k = 0; k < P; ++k)
+    {
+      CT sp = 0, sm = 0;
+#pragma unroll
+      for(int a = 0; a < P; ++a)
+      {
+        sp += S[a * K + k] * rp[a];
+        sm += S[a * K + P + k] * rm[a];
+      }
+      o[k] = sp;
+      o[P + k] = sm;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int a = 0; a < K; ++a) s += S[a * K + i] * r[a];
+      o[i] = s;
+    }
+  }
+}
+
+template<int K, bool EVEN_ODD, typename CT>
+__device__ __forceinline__ void s_backward(const CT *S, const CT *o, CT *r)
+{
+  constexpr int P = K / 2;
+  if constexpr(EVEN_ODD)
+  {
+#pragma unroll
+    for(int a = 0; a < P; ++a)
+    {
+      CT sp = 0, sm = 0;
+#pragma unroll
+      for(int k = 0; k < P; ++k)
+      {
+        sp += S[a * K + k] * o[k];
+        sm += S[a * K + P + k] * o[P + k];
+      }
+      r[a] = sp + sm;
+      r[K - 1 - a] = sp - sm;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for(int a = 0; a < K; ++a)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int i = 0; i < K; ++i) s += S[a * K + i] * o[i];
+      r[a] = s;
+    }
+  }
+}
+
 // Fast-diagonalization inverse of one patch held as x-lines: thread t owns the
 // x-line (y, z) = (t % K, t / K); on entry r is that line of the patch
 // right-hand side, on exit o is the same line of
 //   u = (S_2 x S_1 x S_0) diag(w / (lam_0 + lam_1 + lam_2)) (S_2^T x S_1^T x S_0^T) r.
 // The line direction rotates x -> y -> z -> y -> x through the patch's LDS
 // buffer B (entry (x, y, z) at x + SY y + SZ z: padded strides keep the
-// transposes free of LDS bank conflicts); contains __syncthreads (every thread
-// of the block must call it).
-template<int DIM, int K, typename CT, int SY = K, int SZ = K * K>
+// transposes free of LDS bank conflicts); EO: even-odd products in every
+// direction.  Contains __syncthreads (every thread of the block must call it).
+template<int DIM, int K, typename CT, int SY = K, int SZ = K * K, bool EO = false>
 __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, const CT *S1,
                                          const CT *S2, const CT *l0, const CT *l1, const CT *l2,
                                          const CT w, const int t, const bool owner)
 {
   const int ty = DIM > 1 ? t % K : 0;
   const int tz = DIM > 2 ? t / K : 0;
-  // forward x
-#pragma unroll
-  for(int i = 0; i < K; ++i)
-  {
-    CT s = 0;
-#pragma unroll
-    for(int a = 0; a < K; ++a) s += S0[a * K + i] * r[a];
-    o[i] = s;
-  }
+  s_forward<K, EO>(S0, r, o);
   if(DIM == 2)
   {
 #pragma unroll
@@ -383,22 +455,10 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
     const int x0 = t;
 #pragma unroll
     for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + SY * jj];
+    s_forward<K, EO>(S1, r, o);
 #pragma unroll
-    for(int jj = 0; jj < K; ++jj)
-    {
-      CT s = 0;
-#pragma unroll
-      for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
-      o[jj] = eig_scale(s, w, l0[x0] + l1[jj]);
-    }
-#pragma unroll
-    for(int b = 0; b < K; ++b)
-    {
-      CT s = 0;
-#pragma unroll
-      for(int jj = 0; jj < K; ++jj) s += S1[b * K + jj] * o[jj];
-      r[b] = s;
-    }
+    for(int jj = 0; jj < K; ++jj) o[jj] = eig_scale(o[jj], w, l0[x0] + l1[jj]);
+    s_backward<K, EO>(S1, o, r);
     __syncthreads();
 #pragma unroll
     for(int b = 0; b < K; ++b)
@@ -417,14 +477,7 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
       const int x0 = t % K, z0 = t / K;
 #pragma unroll
       for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + SY * jj + SZ * z0];
-#pragma unroll
-      for(int jj = 0; jj < K; ++jj)
-      {
-        CT s = 0;
-#pragma unroll
-        for(int b = 0; b < K; ++b) s += S1[b * K + jj] * r[b];
-        o[jj] = s;
-      }
+      s_forward<K, EO>(S1, r, o);
 #pragma unroll
       for(int jj = 0; jj < K; ++jj)
         if(owner) B[x0 + SY * jj + SZ * z0] = o[jj];
@@ -435,22 +488,10 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
 #pragma unroll
       for(int l = 0; l < K; ++l) r[l] = B[x0 + SY * y0 + SZ * l];
       const CT lxy = l0[x0] + l1[y0];
+      s_forward<K, EO>(S2, r, o);
 #pragma unroll
-      for(int l = 0; l < K; ++l)
-      {
-        CT s = 0;
-#pragma unroll
-        for(int c = 0; c < K; ++c) s += S2[c * K + l] * r[c];
-        o[l] = eig_scale(s, w, lxy + l2[l]);
-      }
-#pragma unroll
-      for(int c = 0; c < K; ++c)
-      {
-        CT s = 0;
-#pragma unroll
-        for(int l = 0; l < K; ++l) s += S2[c * K + l] * o[l];
-        r[c] = s;
-      }
+      for(int l = 0; l < K; ++l) o[l] = eig_scale(o[l], w, lxy + l2[l]);
+      s_backward<K, EO>(S2, o, r);
 #pragma unroll
       for(int c = 0; c < K; ++c)
         if(owner) B[x0 + SY * y0 + SZ * c] = r[c];
@@ -460,14 +501,7 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
       const int x0 = t % K, z0 = t / K;
 #pragma unroll
       for(int jj = 0; jj < K; ++jj) r[jj] = B[x0 + SY * jj + SZ * z0];
-#pragma unroll
-      for(int b = 0; b < K; ++b)
-      {
-        CT s = 0;
-#pragma unroll
-        for(int jj = 0; jj < K; ++jj) s += S1[b * K + jj] * r[jj];
-        o[b] = s;
-      }
+      s_backward<K, EO>(S1, r, o);
 #pragma unroll
       for(int b = 0; b < K; ++b)
         if(owner) B[x0 + SY * b + SZ * z0] = o[b];
@@ -476,15 +510,7 @@ __device__ __forceinline__ void fd_lines(CT *r, CT *o, CT *B, const CT *S0, cons
 #pragma unroll
     for(int i = 0; i < K; ++i) r[i] = B[i + SY * ty + SZ * tz];
   }
-  // backward x
-#pragma unroll
-  for(int a = 0; a < K; ++a)
-  {
-    CT s = 0;
-#pragma unroll
-    for(int i = 0; i < K; ++i) s += S0[a * K + i] * r[i];
-    o[a] = s;
-  }
+  s_backward<K, EO>(S0, r, o);
 }
 
 template<int DIM, int K, typename VT, typename CT, int MODE>

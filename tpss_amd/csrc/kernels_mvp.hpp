@@ -175,7 +175,8 @@ __global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS), (mvp_min_waves<
 k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, K / 2> tab,
                   const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
                   const CT *__restrict__ S_tab, const CT *__restrict__ lam_tab,
-                  const float *__restrict__ omega, const CT relaxation, const int patch_begin,
+                  const uint32_t *__restrict__ eo_tab, const float *__restrict__ omega,
+                  const CT relaxation, const int patch_begin,
                   const int patch_end, const int n, const int x_is_zero)
 {
   using PS = PatchShape<DIM, K>;
@@ -207,8 +208,8 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   // scalar registers (operands of the contraction FMAs) instead of being
   // staged through LDS, which takes most LDS reads off the pass.  (FP64
   // matrices would need twice the scalar registers and spill.)
-  constexpr bool S_SCALAR =
-      FD == 0 && std::is_same<CT, float>::value && L == 64 && PS::THREADS == PPB * 64;
+  constexpr bool WAVE_PATCH = FD == 0 && L == 64 && PS::THREADS == PPB * 64;
+  constexpr bool S_SCALAR = WAVE_PATCH && std::is_same<CT, float>::value;
   if constexpr(!S_SCALAR)
     for(int idx = threadIdx.x; idx < PPB * DIM * K * K; idx += blockDim.x)
     {
@@ -431,17 +432,25 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   {
 #pragma unroll
     for(int i = 0; i < K; ++i) r[i] = g[i];
-    if constexpr(S_SCALAR)
+    if constexpr(WAVE_PATCH)
     {
       // wave index as a wave-uniform value; the last patch of the pass stands
       // in for a missing one (its results are discarded)
       const int pw = __builtin_amdgcn_readfirstlane(int(threadIdx.x)) / 64;
       const int jw = min(j0 + pw, patch_end - 1);
-      const CT *S0 = S_tab + size_t(fac[size_t(jw) * DIM + 0]) * K * K;
-      const CT *S1 = S_tab + size_t(fac[size_t(jw) * DIM + 1]) * K * K;
-      const CT *S2 = S_tab + size_t(fac[size_t(jw) * DIM + 2]) * K * K;
-      fd_lines<DIM, K, CT, FSY, FSZ>(r, o, PAD ? A2[plc] : buf[plc], S0, S1, S2, ls[plc][0],
-                                     ls[plc][1], ls[plc][2], w, t, owner);
+      const uint32_t f0 = fac[size_t(jw) * DIM + 0], f1 = fac[size_t(jw) * DIM + 1],
+                     f2 = fac[size_t(jw) * DIM + 2];
+      const CT *S0 = S_SCALAR ? S_tab + size_t(f0) * K * K : Ss[plc][0];
+      const CT *S1 = S_SCALAR ? S_tab + size_t(f1) * K * K : Ss[plc][1];
+      const CT *S2 = S_SCALAR ? S_tab + size_t(f2) * K * K : Ss[plc][2];
+      // even-odd products where all three 1D problems are mirror-symmetric
+      // (interior patches of a uniform mesh); a wave-uniform branch
+      if(eo_tab && (eo_tab[f0] & eo_tab[f1] & eo_tab[f2]) != 0u)
+        fd_lines<DIM, K, CT, FSY, FSZ, true>(r, o, PAD ? A2[plc] : buf[plc], S0, S1, S2,
+                                             ls[plc][0], ls[plc][1], ls[plc][2], w, t, owner);
+      else
+        fd_lines<DIM, K, CT, FSY, FSZ, false>(r, o, PAD ? A2[plc] : buf[plc], S0, S1, S2,
+                                              ls[plc][0], ls[plc][1], ls[plc][2], w, t, owner);
     }
     else
       fd_lines<DIM, K, CT, FSY, FSZ>(r, o, PAD ? A2[plc] : buf[plc], Ss[plc][0], Ss[plc][1],

@@ -256,6 +256,65 @@ k_check_eigenvalue_sums(const uint32_t *__restrict__ fac, const double *__restri
   if(!(small > 1e-12 * big)) status[0] = 1;
 }
 
+// Eigenpairs of mirror-symmetric 1D problems (two cells of equal length, no
+// boundary face: M and L commute with the reversal a -> K-1-a) are K/2
+// symmetric and K/2 antisymmetric eigenvectors.  For those the pairs are
+// reordered symmetric first (each group in ascending eigenvalue order), the
+// symmetry is made exact, and eo[f] = 1, so that patch solves can take the
+// even-odd products.  Any other problem (boundary face, unequal lengths, a
+// degenerate pair mixing the two kinds) keeps its order and eo[f] = 0.  The
+// inverse (S diag(1/lam) S^T in tensor form) does not depend on the order.
+template<int K>
+__global__ void __launch_bounds__(64)
+k_even_odd_order(double *__restrict__ S_tab, double *__restrict__ lam_tab, const int nf,
+                 uint32_t *__restrict__ eo)
+{
+  constexpr int P = K / 2;
+  const int f = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(f >= nf) return;
+  double *S = S_tab + size_t(f) * K * K;
+  double *lam = lam_tab + size_t(f) * K;
+  int kind[K];
+  int nsym = 0, nanti = 0;
+  for(int c = 0; c < K; ++c)
+  {
+    double nrm = 0.0, ds = 0.0, da = 0.0;
+    for(int a = 0; a < K; ++a) nrm = fmax(nrm, fabs(S[a * K + c]));
+    for(int a = 0; a < P; ++a)
+    {
+      ds = fmax(ds, fabs(S[a * K + c] - S[(K - 1 - a) * K + c]));
+      da = fmax(da, fabs(S[a * K + c] + S[(K - 1 - a) * K + c]));
+    }
+    kind[c] = ds <= 1e-11 * nrm ? 1 : (da <= 1e-11 * nrm ? -1 : 0);
+    nsym += kind[c] == 1 ? 1 : 0;
+    nanti += kind[c] == -1 ? 1 : 0;
+  }
+  if(nsym != P || nanti != P)
+  {
+    eo[f] = 0u;
+    return;
+  }
+  double Sn[K * K], ln[K];
+  int m = 0;
+  for(int pass = 0; pass < 2; ++pass)
+    for(int c = 0; c < K; ++c)
+    {
+      const int sg = kind[c];
+      if(sg != (pass == 0 ? 1 : -1)) continue;
+      ln[m] = lam[c];
+      for(int a = 0; a < P; ++a)
+      {
+        const double v = 0.5 * (S[a * K + c] + sg * S[(K - 1 - a) * K + c]);
+        Sn[a * K + m] = v;
+        Sn[(K - 1 - a) * K + m] = sg * v;
+      }
+      ++m;
+    }
+  for(int e = 0; e < K * K; ++e) S[e] = Sn[e];
+  for(int c = 0; c < K; ++c) lam[c] = ln[c];
+  eo[f] = 1u;
+}
+
 // Surrogate lengths (A9, mapping_info.templates.h:42-240): per patch, direction
 // and 1D cell slot the average over the 2^(dim-1) cells of that slot of the
 // Gauss-Lobatto weighted distance between opposite faces along reference lines

@@ -363,6 +363,14 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv,
       if(status[1]) throw Error(TPSS_E_SINGULAR, "local solver setup: fitted solver has zero eigenvalues");
     }
   }
+  // symmetric-first order for mirror-symmetric problems (even-odd solves)
+  ls->d_eo.resize(size_t(nf));
+  dispatch_dim_np(dim, lv->np, [&](auto, auto NPc) {
+    constexpr int NP = decltype(NPc)::value;
+    hipLaunchKernelGGL((k_even_odd_order<2 * NP>), dim3(blocks_for(nf, 64)), dim3(64), 0, s,
+                       ls->S64.ptr, ls->lam64.ptr, nf, ls->d_eo.ptr);
+    TPSS_HIP(hipGetLastError());
+  });
   ls->S32.resize(ls->S64.n);
   ls->lam32.resize(ls->lam64.n);
   convert(ls->S32.ptr, TPSS_F32, ls->S64.ptr, TPSS_F64, int64_t(ls->S64.n), s);
@@ -462,15 +470,15 @@ void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, b
     {
       if constexpr(DIM == 3 && K == 8 && std::is_same<CT, float>::value)
         hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT, 1>), dim3(grid), dim3(PS::THREADS), 0,
-                           s, b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end,
-                           ls.n, x_is_zero ? 1 : 0);
+                           s, b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, nullptr, om, relax,
+                           begin, end, ls.n, x_is_zero ? 1 : 0);
       else
         throw Error(TPSS_E_UNSUPPORTED, "BF16-MFMA patch solves are built for 3D Q3 (k = 8)");
     }
     else
       hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT>), dim3(grid), dim3(PS::THREADS), 0, s,
-                         b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax, begin, end, ls.n,
-                         x_is_zero ? 1 : 0);
+                         b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, ls.d_eo.ptr, om, relax,
+                         begin, end, ls.n, x_is_zero ? 1 : 0);
     TPSS_HIP(hipGetLastError());
   });
 }

@@ -29,6 +29,7 @@ struct LocalSolvers
   std::vector<double> prob_coef;   // nfac * 2: (a_cons, a_pen) of the 1D matrix
   DeviceArray<double> S64, lam64;  // nfac * K * K, nfac * K
   DeviceArray<float> S32, lam32;
+  DeviceArray<uint32_t> d_eo;      // nfac: 1 = eigenpairs in even-odd order
   std::vector<double> h_host;      // P * dim * 2 surrogate lengths
   std::vector<double> fit_err;     // P: relative error of the separable fit (-1: surrogate kept)
   std::vector<std::shared_ptr<DeviceArray<uint32_t>>> color_cells;
