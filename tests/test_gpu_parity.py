@@ -237,6 +237,24 @@ def test_distorted_operator_parity(dim, p, n):
     assert rel_err(host(y), b - ref.operator(x)) < 1e-12
 
 
+@pytest.mark.parametrize("dim,p,n", [(2, 3, 4), (3, 2, 3), (3, 3, 4), (3, 4, 3)])
+def test_distorted_operator_parity_f32(dim, p, n):
+    """FP32 levels evaluate the moved-vertex operator in FP32 arithmetic (the
+    Jacobians too, from map coefficients formed in FP64): max-norm error within
+    1e-5 of the FP64 oracle, relative, for the operator and the residual."""
+    t = _tpss()
+    V = o.distorted_vertices(dim, n, 0.2, seed=3)
+    lv = t.Level(dim, p, n, ip_factor=4.0, vertices=V, precision=t.F32)
+    ref = o.Smoother(dim, p, n, ip=4.0, vertices=V)
+    x = o.splitmix_uniform(1, ref.N).astype(np.float32).astype(np.float64)
+    y = dev(np.zeros(ref.N), t.F32)
+    lv.vmult(y, dev(x, t.F32))
+    assert rel_err(host(y), ref.operator(x)) < 1e-5
+    b = o.splitmix_uniform(2, ref.N).astype(np.float32).astype(np.float64)
+    lv.residual(y, dev(b, t.F32), dev(x, t.F32))
+    assert rel_err(host(y), b - ref.operator(x)) < 1e-5
+
+
 @pytest.mark.parametrize("dim,p,n,variant", [(2, 3, 6, 0), (3, 3, 4, 0), (3, 2, 5, 1), (3, 4, 3, 0)])
 def test_distorted_smoother_parity(dim, p, n, variant):
     V = o.distorted_vertices(dim, n, 0.15, seed=3)

@@ -107,23 +107,23 @@ void operator_apply(const tpss_level *lv, VT *y, const VT *x, const VT *b, const
     dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
       constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
       constexpr int NDC = cpow(NP, DIM);
-      GeneralTab<NP> tab;
+      GeneralTab<NP, VT> tab;
       const Basis1D bs = make_basis(NP - 1);
       for(int q = 0; q < NP; ++q)
       {
-        tab.qx[q] = bs.qx[q];
-        tab.qw[q] = bs.qw[q];
+        tab.qx[q] = VT(bs.qx[q]);
+        tab.qw[q] = VT(bs.qw[q]);
         for(int i = 0; i < NP; ++i)
         {
-          tab.V[q * NP + i] = bs.val[i * NP + q];
-          tab.Dv[q * NP + i] = bs.grad[i * NP + q];
+          tab.V[q * NP + i] = VT(bs.val[i * NP + q]);
+          tab.Dv[q * NP + i] = VT(bs.grad[i * NP + q]);
         }
       }
       for(int f = 0; f < 2; ++f)
         for(int i = 0; i < NP; ++i)
         {
-          tab.f[f][i] = bs.fval[i * 2 + f];
-          tab.fd[f][i] = bs.fgrad[i * 2 + f];
+          tab.f[f][i] = VT(bs.fval[i * 2 + f]);
+          tab.fd[f][i] = VT(bs.fgrad[i * 2 + f]);
         }
       const int64_t count = cells ? n_list : lv->ncell;
       if(count == 0) return;

@@ -29,6 +29,21 @@
 
 namespace tpss
 {
+// contraction of an LDS tensor along direction e: out(t) = sum_j M[o*NP + j] src(t|e->j),
+// o = t_e; with TRANS the matrix is read as M[j*NP + o]
+template<int DIM, int NP, bool TRANS>
+__device__ __forceinline__ double contract(const double *src, const double *M, const int *tid,
+                                           const int t, const int e)
+{
+  constexpr int ST[3] = {1, NP, NP * NP};
+  const int o = tid[e];
+  const int base = t - o * ST[e];
+  double s = 0.0;
+#pragma unroll
+  for(int j = 0; j < NP; ++j) s += (TRANS ? M[j * NP + o] : M[o * NP + j]) * src[base + j * ST[e]];
+  return s;
+}
+
 template<int NP>
 struct ElastTab
 {

@@ -2,13 +2,27 @@
 // ((bi/tri)linear cells), matrix-free with QGauss(p+1) on cells and faces:
 // Laplace::DG::MF::Operator::{apply_cell, apply_face, apply_boundary}
 // (apps/laplace_integrator.h:613-743).  The geometry (Jacobians at quadrature
-// points) is recomputed from the cell vertices on the fly, so the only HBM
-// traffic is the vectors and the (n+1)^dim vertex array.
+// points) is recomputed on the fly from the multilinear coefficients of each
+// cell's map, so the only HBM traffic is the vectors and the (n+1)^dim vertex
+// array.
 //
-// One block per cell, one thread per cell DoF / quadrature point (nq = np).
-// Faces are evaluated element-centrically: every cell integrates its own test
-// functions against the flux of each of its 2*dim faces (interior faces are
-// thus evaluated from both sides, which is what makes the kernel write-race free).
+// One block per cell, one thread per cell DoF / quadrature point (nq = np);
+// arithmetic in the vector type (FP32 levels compute in FP32, FP64 levels in
+// FP64; the map coefficients are formed in FP64 from the vertices).  Faces are
+// evaluated element-centrically: every cell integrates its own test functions
+// against the flux of each of its 2*dim faces (interior faces are evaluated
+// from both sides, which keeps the kernel free of write races).
+//
+// Every contraction is sum-factorised, and independent work shares barriers:
+//   P0 load u, the face neighbours and the corners of the cell and neighbours
+//   P1 x-interpolation of u (value, derivative); map coefficients
+//   P2 y-interpolation; face traces along the normals (value, normal derivative)
+//   P3 z-interpolation -> reference gradient -> geometry -> reference flux;
+//      face traces along the first tangential direction
+//   P4 z-integration; face traces along the second tangential direction,
+//      face geometry (own and neighbour Jacobians) and fluxes per face point
+//   P5 y-integration; face fluxes projected on the tangential test functions
+//   P6 x-integration; add the face contributions of this test function
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,419 +33,527 @@
 
 namespace tpss
 {
-template<int NP>
+template<int NP, typename T>
 struct GeneralTab
 {
-  double V[NP * NP];   // V[q*NP + i]  = phi_i(x_q), Gauss points x_q
-  double Dv[NP * NP];  // Dv[q*NP + i] = phi_i'(x_q)
-  double qx[NP], qw[NP];
-  double f[2][NP];     // phi_i(0), phi_i(1)
-  double fd[2][NP];    // phi_i'(0), phi_i'(1)
+  T V[NP * NP];   // V[q*NP + i]  = phi_i(x_q), Gauss points x_q
+  T Dv[NP * NP];  // Dv[q*NP + i] = phi_i'(x_q)
+  T qx[NP], qw[NP];
+  T f[2][NP];     // phi_i(0), phi_i(1)
+  T fd[2][NP];    // phi_i'(0), phi_i'(1)
 };
 
-// J[a][b] = d x_a / d xi_b of the multilinear map with corner coordinates X
-template<int DIM>
-__device__ __forceinline__ void map_jacobian(const double (*X)[3], const double *xi,
-                                             double (*J)[3])
+// Jacobian J[a][b] = d x_a / d xi_b of the multilinear map
+//   x(xi) = sum_S A[S] prod_{e in S} xi_e   (S: subsets of the directions, bit masks)
+template<int DIM, typename T>
+__device__ __forceinline__ void coef_jacobian(const T (*A)[3], const T *xi, T (*J)[3])
 {
+#pragma unroll
   for(int a = 0; a < DIM; ++a)
-    for(int b = 0; b < DIM; ++b) J[a][b] = 0.0;
-  for(int c = 0; c < (1 << DIM); ++c)
-  {
+#pragma unroll
+    for(int b = 0; b < DIM; ++b) J[a][b] = T(0);
+#pragma unroll
+  for(int S = 1; S < (1 << DIM); ++S)
+#pragma unroll
     for(int b = 0; b < DIM; ++b)
     {
-      double w = 1.0;
+      if(!((S >> b) & 1)) continue;
+      T w = T(1);
+#pragma unroll
       for(int e = 0; e < DIM; ++e)
-      {
-        const int bit = (c >> e) & 1;
-        if(e == b)
-          w *= bit ? 1.0 : -1.0;
-        else
-          w *= bit ? xi[e] : 1.0 - xi[e];
-      }
-      for(int a = 0; a < DIM; ++a) J[a][b] += w * X[c][a];
+        if(e != b && ((S >> e) & 1)) w *= xi[e];
+#pragma unroll
+      for(int a = 0; a < DIM; ++a) J[a][b] += w * A[S][a];
     }
-  }
 }
 
 // Ji = J^{-1}, returns det J
-template<int DIM>
-__device__ __forceinline__ double invert(const double (*J)[3], double (*Ji)[3])
+template<int DIM, typename T>
+__device__ __forceinline__ T invert(const T (*J)[3], T (*Ji)[3])
 {
-  if(DIM == 2)
+  if constexpr(DIM == 2)
   {
-    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    Ji[0][0] = J[1][1] / det;
-    Ji[0][1] = -J[0][1] / det;
-    Ji[1][0] = -J[1][0] / det;
-    Ji[1][1] = J[0][0] / det;
+    const T det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const T id = T(1) / det;
+    Ji[0][0] = J[1][1] * id;
+    Ji[0][1] = -J[0][1] * id;
+    Ji[1][0] = -J[1][0] * id;
+    Ji[1][1] = J[0][0] * id;
     return det;
   }
-  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-  const double id = 1.0 / det;
-  Ji[0][0] = c00 * id;
-  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-  Ji[1][0] = c01 * id;
-  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-  Ji[2][0] = c02 * id;
-  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-  return det;
-}
-
-template<int DIM>
-__device__ __forceinline__ void load_corners(const double *__restrict__ vert, const int n,
-                                             const int *cc, double (*X)[3], int lane)
-{
-  // lanes 0 .. 2^DIM*DIM-1 load one coordinate each
-  if(lane < (1 << DIM) * DIM)
+  else
   {
-    const int c = lane / DIM, a = lane % DIM;
-    int64_t idx = 0, st = 1;
-    for(int e = 0; e < DIM; ++e)
-    {
-      idx += int64_t(cc[e] + ((c >> e) & 1)) * st;
-      st *= (n + 1);
-    }
-    X[c][a] = vert[idx * DIM + a];
+    const T c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const T det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const T id = T(1) / det;
+    Ji[0][0] = c00 * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][0] = c01 * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][0] = c02 * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    return det;
   }
-}
-
-// contraction of an LDS tensor along direction e: out(t) = sum_j M[o*NP + j] src(t|e->j),
-// o = t_e; with TRANS the matrix is read as M[j*NP + o]
-template<int DIM, int NP, bool TRANS>
-__device__ __forceinline__ double contract(const double *src, const double *M, const int *tid,
-                                           const int t, const int e)
-{
-  constexpr int ST[3] = {1, NP, NP * NP};
-  const int o = tid[e];
-  const int base = t - o * ST[e];
-  double s = 0.0;
-#pragma unroll
-  for(int j = 0; j < NP; ++j) s += (TRANS ? M[j * NP + o] : M[o * NP + j]) * src[base + j * ST[e]];
-  return s;
 }
 
 template<int DIM, int NP, typename VT, bool RESIDUAL>
-__global__ void __launch_bounds__(128, 2)
+__global__ void __launch_bounds__((cpow(NP, DIM) <= 64 ? 64 : 128))
 k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
-               const double *__restrict__ vert, const GeneralTab<NP> tab, const int n,
+               const double *__restrict__ vert, const GeneralTab<NP, VT> tab, const int n,
                const double ip, const int p, const uint32_t *__restrict__ cells)
 {
+  using T = VT;
   constexpr int NDC = cpow(NP, DIM);
   constexpr int NF = cpow(NP, DIM - 1);
   constexpr int NFACE = 2 * DIM;
-  __shared__ double U[NDC], A[NDC], B[NDC];
-  __shared__ double UN[NFACE][NDC];  // the face neighbours (face f = 2 d + side)
-  __shared__ double F[DIM][NDC];
-  __shared__ double X[8][3], XN[NFACE][8][3];
-  __shared__ double FV[NFACE][NF], FH[NFACE][DIM][NF];
-  // traces contracted along each face normal: value (PA) and normal derivative
-  // (PB) of the own cell [0] and the neighbour [1] on the NF tangential nodes
-  __shared__ double PA[NFACE][2][NF], PB[NFACE][2][NF];
-  __shared__ double HP[NFACE][2];  // penalty lengths of the own cell and the neighbour
-  __shared__ GeneralTab<NP> tl;  // the 1D tables, indexed at run time
+  constexpr int NCORN = 1 << DIM;
+  constexpr int NT = NDC <= 64 ? 64 : 128;  // block size
+  constexpr int ST[3] = {1, NP, NP * NP};
+  constexpr int NP2 = DIM == 3 ? NP : 1;    // tangential nodes along the second face direction
+
+  __shared__ T U[NDC];
+  __shared__ T A0[NDC], A1[NDC], A2[NDC];   // x-stage (gradient) / F (flux) / y-stage (integration)
+  __shared__ T B0[NDC], B1[NDC], B2[NDC];   // y-stage (gradient) / z-stage (integration)
+  __shared__ T PA[NFACE][2][NF], PB[NFACE][2][NF];  // normal traces; later the projections
+  __shared__ T C[NFACE + 1][NCORN][3];      // map coefficients: own cell [0], neighbours [1 + f]
+  __shared__ T HP[NFACE][2];
+  __shared__ GeneralTab<NP, T> tl;
+  // storage shared by arrays with disjoint lifetimes: the face neighbours (P0-P2)
+  // with the face fluxes (P4-P5); the corners (P0-P1) with the tangential traces (P3-P4)
+  constexpr int NUN = NFACE * NDC, NFLUX = NFACE * NF * (2 + DIM);
+  constexpr int NTR = 3 * NFACE * 2 * NF, NXD = (NFACE + 1) * NCORN * 3;
+  __shared__ T RU[NUN > NFLUX ? NUN : NFLUX];
+  __shared__ double RX[(NTR * int(sizeof(T)) + 7) / 8 > NXD ? (NTR * int(sizeof(T)) + 7) / 8 : NXD];
+  T(*UN)[NDC] = reinterpret_cast<T(*)[NDC]>(RU);
+  T(*JV)[NF] = reinterpret_cast<T(*)[NF]>(RU);
+  T(*AV)[NF] = reinterpret_cast<T(*)[NF]>(RU + NFACE * NF);
+  T(*FH)[DIM][NF] = reinterpret_cast<T(*)[DIM][NF]>(RU + 2 * NFACE * NF);
+  double(*X)[NCORN][3] = reinterpret_cast<double(*)[NCORN][3]>(RX);
+  T(*VA)[2][NF] = reinterpret_cast<T(*)[2][NF]>(RX);
+  T(*DA)[2][NF] = reinterpret_cast<T(*)[2][NF]>(reinterpret_cast<T *>(RX) + NFACE * 2 * NF);
+  T(*VB)[2][NF] = reinterpret_cast<T(*)[2][NF]>(reinterpret_cast<T *>(RX) + 2 * NFACE * 2 * NF);
 
   const int64_t cell = cells ? int64_t(cells[blockIdx.x]) : int64_t(blockIdx.x);
   const int t = threadIdx.x;
-  if(t == 0) tl = tab;
   const bool act = t < NDC;
   const int ts = act ? t : 0;
   int cc[3] = {0, 0, 0};
   {
     int64_t r = cell;
+#pragma unroll
     for(int d = 0; d < DIM; ++d)
     {
       cc[d] = int(r % n);
       r /= n;
     }
   }
-  int tid[3] = {0, 0, 0};
-  {
-    int r = ts;
-    for(int d = 0; d < DIM; ++d)
-    {
-      tid[d] = r % NP;
-      r /= NP;
-    }
-  }
-  if(act) U[t] = double(x[cell * NDC + t]);
-  load_corners<DIM>(vert, n, cc, X, t);
-  __syncthreads();
-
-  // ---------------- cell term: reference gradient at the quadrature point t
-  double g[3] = {0, 0, 0};
-#pragma unroll
-  for(int a = 0; a < DIM; ++a)
-  {
-    // contract along every direction e with Dv (e == a) or V (e != a)
-    const double *src = U;
-#pragma unroll
-    for(int e = 0; e < DIM; ++e)
-    {
-      const double v = contract<DIM, NP, false>(src, e == a ? tl.Dv : tl.V, tid, ts, e);
-      double *dst = (e % 2 == 0) ? A : B;
-      __syncthreads();
-      if(act) dst[t] = v;
-      __syncthreads();
-      src = dst;
-    }
-    g[a] = src[ts];
-    __syncthreads();
-  }
-  {
-    double xi[3], J[3][3], Ji[3][3];
-    double w = 1.0;
-    for(int e = 0; e < DIM; ++e)
-    {
-      xi[e] = tl.qx[tid[e]];
-      w *= tl.qw[tid[e]];
-    }
-    map_jacobian<DIM>(X, xi, J);
-    const double det = invert<DIM>(J, Ji);
-    const double jxw = fabs(det) * w;
-    // flux in reference coordinates: Ji Ji^T g |det J| w
-    double ph[3];
-    for(int c = 0; c < DIM; ++c)
-    {
-      double s = 0.0;
-      for(int bb = 0; bb < DIM; ++bb) s += Ji[bb][c] * g[bb];
-      ph[c] = s;  // physical gradient
-    }
-    for(int a = 0; a < DIM; ++a)
-    {
-      double s = 0.0;
-      for(int c = 0; c < DIM; ++c) s += Ji[a][c] * ph[c];
-      if(act) F[a][t] = s * jxw;
-    }
-  }
-  __syncthreads();
-  double acc = 0.0;
-#pragma unroll
-  for(int a = 0; a < DIM; ++a)
-  {
-    const double *src = F[a];
-#pragma unroll
-    for(int e = 0; e < DIM; ++e)
-    {
-      const double v = contract<DIM, NP, true>(src, e == a ? tl.Dv : tl.V, tid, ts, e);
-      double *dst = (e % 2 == 0) ? A : B;
-      __syncthreads();
-      if(act) dst[t] = v;
-      __syncthreads();
-      src = dst;
-    }
-    acc += src[ts];
-    __syncthreads();
-  }
-
-  // ---------------- face terms, all 2 DIM faces of the cell at once
-  const double gamma = p == 0 ? 1.0 : double(p) * (p + 1);
-  auto face_interior = [&](int f) {
+  const int ix = ts % NP, iy = DIM > 1 ? (ts / NP) % NP : 0, iz = DIM > 2 ? ts / (NP * NP) : 0;
+  auto interior = [&](int f) {
     const int d = f / 2;
     return (f % 2) == 1 ? cc[d] < n - 1 : cc[d] > 0;
   };
-  {
+  auto nb_stride = [&](int d) {
     int64_t st = 1;
+    for(int e = 0; e < d; ++e) st *= n;
+    return st;
+  };
+
+  // ---------------- P0: loads
+  {
+    const T *src = reinterpret_cast<const T *>(&tab);
+    T *dst = reinterpret_cast<T *>(&tl);
+    constexpr int NTAB = int(sizeof(GeneralTab<NP, T>) / sizeof(T));
+    for(int i = t; i < NTAB; i += NT) dst[i] = src[i];
+  }
+  if(act)
+  {
+    U[t] = x[cell * NDC + t];
+#pragma unroll
     for(int f = 0; f < NFACE; ++f)
+      if(interior(f))
+      {
+        const int64_t st = nb_stride(f / 2);
+        UN[f][t] = x[(f % 2 == 1 ? cell + st : cell - st) * NDC + t];
+      }
+  }
+  for(int it = t; it < (NFACE + 1) * NCORN * DIM; it += NT)
+  {
+    const int k = it / (NCORN * DIM), c = (it / DIM) % NCORN, a = it % DIM;
+    if(k > 0 && !interior(k - 1)) continue;
+    int cn[3] = {cc[0], cc[1], cc[2]};
+    if(k > 0)
     {
-      const int d = f / 2, side = f % 2;
-      if(d > 0 && side == 0) st *= n;
-      if(!face_interior(f)) continue;
-      const int64_t nb = side == 1 ? cell + st : cell - st;
-      int cn[3] = {cc[0], cc[1], cc[2]};
-      cn[d] += side == 1 ? 1 : -1;
-      if(act) UN[f][t] = double(x[nb * NDC + t]);
-      load_corners<DIM>(vert, n, cn, XN[f], t);
+      const int f = k - 1;
+      cn[f / 2] += f % 2 == 1 ? 1 : -1;
     }
+    int64_t idx = 0, st = 1;
+    for(int e = 0; e < DIM; ++e)
+    {
+      idx += int64_t(cn[e] + ((c >> e) & 1)) * st;
+      st *= (n + 1);
+    }
+    X[k][c][a] = vert[idx * DIM + a];
   }
   __syncthreads();
-  constexpr int ST[3] = {1, NP, NP * NP};
-  // contraction along the normal: every face, both sides, all tangential nodes
-  for(int o = t; o < NFACE * 2 * NF; o += blockDim.x)
+
+  // ---------------- P1: x-interpolation; map coefficients
+  if(act)
+  {
+    T v = 0, g = 0;
+    const int base = ts - ix;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+    {
+      const T u = U[base + j];
+      v += tl.V[ix * NP + j] * u;
+      g += tl.Dv[ix * NP + j] * u;
+    }
+    A0[t] = v;
+    A1[t] = g;
+  }
+  for(int it = t; it < (NFACE + 1) * NCORN * DIM; it += NT)
+  {
+    // A_S = sum over subsets R of S of (-1)^{|S \ R|} X_R
+    const int k = it / (NCORN * DIM), S = (it / DIM) % NCORN, a = it % DIM;
+    if(k > 0 && !interior(k - 1)) continue;
+    double s = 0.0;
+    for(int R = 0; R < NCORN; ++R)
+      if((R & S) == R) s += (__popc(unsigned(S ^ R)) & 1) ? -X[k][R][a] : X[k][R][a];
+    C[k][S][a] = T(s);
+  }
+  __syncthreads();
+
+  // ---------------- P2: y-interpolation; traces along the face normals
+  T g0 = 0, g1 = 0, g2 = 0;
+  if(act)
+  {
+    const int base = ts - iy * NP;
+    T b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+    {
+      const T a0 = A0[base + j * NP], a1 = A1[base + j * NP];
+      const T vy = tl.V[iy * NP + j], dy = tl.Dv[iy * NP + j];
+      b0 += vy * a1;
+      b1 += dy * a0;
+      b2 += vy * a0;
+    }
+    if constexpr(DIM == 2)
+    {
+      g0 = b0;
+      g1 = b1;
+    }
+    else
+    {
+      B0[t] = b0;
+      B1[t] = b1;
+      B2[t] = b2;
+    }
+  }
+  for(int o = t; o < NFACE * 2 * NF; o += NT)
   {
     const int f = o / (2 * NF), sd = (o / NF) % 2, m = o % NF;
-    if(sd == 1 && !face_interior(f)) continue;
+    if(sd == 1 && !interior(f)) continue;
     const int d = f / 2, side = f % 2;
     const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
     const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
     const int base = i1 * ST[e1] + (DIM == 3 ? i2 * ST[e2] : 0);
-    const double *src = sd == 0 ? U : UN[f];
+    const T *src = sd == 0 ? U : UN[f];
     const int fs = sd == 0 ? side : 1 - side;
-    double a = 0.0, bsum = 0.0;
+    T a = 0, bs = 0;
 #pragma unroll
     for(int id = 0; id < NP; ++id)
     {
-      const double u = src[base + id * ST[d]];
+      const T u = src[base + id * ST[d]];
       a += tl.f[fs][id] * u;
-      bsum += tl.fd[fs][id] * u;
+      bs += tl.fd[fs][id] * u;
     }
     PA[f][sd][m] = a;
-    PB[f][sd][m] = bsum;
-  }
-  // penalty lengths h = 1/|(J^{-1} n)_d| at the first face quadrature point of
-  // both cells (apps/laplace_integrator.h:651-656, 692-693), one thread per face
-  if(t < NFACE)
-  {
-    const int f = t, d = f / 2, side = f % 2;
-    double xi0[3], J0[3][3], Ji0[3][3], n0[3], nn0 = 0.0;
-    for(int e = 0; e < DIM; ++e) xi0[e] = e == d ? double(side) : tl.qx[0];
-    map_jacobian<DIM>(X, xi0, J0);
-    invert<DIM>(J0, Ji0);
-    for(int a = 0; a < DIM; ++a)
-    {
-      n0[a] = Ji0[d][a] * (side == 0 ? -1.0 : 1.0);
-      nn0 += n0[a] * n0[a];
-    }
-    nn0 = sqrt(nn0);
-    double sdot = 0.0;
-    for(int a = 0; a < DIM; ++a)
-    {
-      n0[a] /= nn0;
-      sdot += Ji0[d][a] * n0[a];
-    }
-    HP[f][0] = 1.0 / fabs(sdot);
-    HP[f][1] = 1.0;
-    if(face_interior(f))
-    {
-      xi0[d] = double(1 - side);
-      map_jacobian<DIM>(XN[f], xi0, J0);
-      invert<DIM>(J0, Ji0);
-      sdot = 0.0;
-      for(int a = 0; a < DIM; ++a) sdot += Ji0[d][a] * n0[a];
-      HP[f][1] = 1.0 / fabs(sdot);
-    }
+    PB[f][sd][m] = bs;
   }
   __syncthreads();
-  // fluxes at every face quadrature point
-  for(int pt = t; pt < NFACE * NF; pt += blockDim.x)
+
+  // ---------------- P3: z-interpolation, geometry, reference flux; tangential
+  // traces along the first face direction
+  if(act)
   {
-    const int f = pt / NF, l = pt % NF;
-    const int d = f / 2, side = f % 2, nside = 1 - side;
-    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
-    const bool interior = face_interior(f);
-    const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
-    double xio[3], xin[3];
-    const double wq = tl.qw[l1] * (DIM == 3 ? tl.qw[l2] : 1.0);
-    for(int e = 0; e < DIM; ++e)
+    if constexpr(DIM == 3)
     {
-      const double qe = e == e1 ? tl.qx[l1] : tl.qx[l2];
-      xio[e] = e == d ? double(side) : qe;
-      xin[e] = e == d ? double(nside) : qe;
-    }
-    // traces: value, normal and the two tangential reference derivatives
-    double uo = 0, un = 0, gon = 0, got1 = 0, got2 = 0, gnn = 0, gnt1 = 0, gnt2 = 0;
-    for(int m = 0; m < NF; ++m)
-    {
-      const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-      const double t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
-      const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
-      const double t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : 0.0;
-      const double wvv = t1v * t2v, wdv = t1d * t2v, wvd = t1v * t2d;
-      const double a0 = PA[f][0][m];
-      uo += wvv * a0;
-      gon += wvv * PB[f][0][m];
-      got1 += wdv * a0;
-      got2 += wvd * a0;
-      if(interior)
+      const int base = ts - iz * NP * NP;
+#pragma unroll
+      for(int j = 0; j < NP; ++j)
       {
-        const double a1 = PA[f][1][m];
-        un += wvv * a1;
-        gnn += wvv * PB[f][1][m];
-        gnt1 += wdv * a1;
-        gnt2 += wvd * a1;
+        const T vz = tl.V[iz * NP + j], dz = tl.Dv[iz * NP + j];
+        g0 += vz * B0[base + j * NP * NP];
+        g1 += vz * B1[base + j * NP * NP];
+        g2 += dz * B2[base + j * NP * NP];
       }
     }
-    double J[3][3], Jio[3][3];
-    map_jacobian<DIM>(X, xio, J);
-    const double deto = invert<DIM>(J, Jio);
-    // outward unit normal of this cell and surface element (Nanson)
-    double nv[3] = {0, 0, 0}, nn = 0.0;
+    T xi[3] = {tl.qx[ix], tl.qx[iy], DIM == 3 ? tl.qx[iz] : T(0)};
+    T J[3][3], Ji[3][3];
+    coef_jacobian<DIM>(C[0], xi, J);
+    const T det = invert<DIM>(J, Ji);
+    const T jxw = fabs(det) * tl.qw[ix] * tl.qw[iy] * (DIM == 3 ? tl.qw[iz] : T(1));
+    const T g[3] = {g0, g1, g2};
+    T ph[3];
+#pragma unroll
+    for(int c = 0; c < DIM; ++c)
+    {
+      T s = 0;
+#pragma unroll
+      for(int bb = 0; bb < DIM; ++bb) s += Ji[bb][c] * g[bb];
+      ph[c] = s;  // physical gradient
+    }
+    T F[3];
+#pragma unroll
     for(int a = 0; a < DIM; ++a)
     {
-      nv[a] = Jio[d][a] * (side == 0 ? -1.0 : 1.0);
+      T s = 0;
+#pragma unroll
+      for(int c = 0; c < DIM; ++c) s += Ji[a][c] * ph[c];
+      F[a] = s * jxw;
+    }
+    // F_a into the x-stage arrays (last read in P2)
+    A0[t] = F[0];
+    A1[t] = F[1];
+    if(DIM == 3) A2[t] = F[2];
+  }
+  for(int o = t; o < NFACE * 2 * NF; o += NT)
+  {
+    // (f, side, l1, i2): along e1 from the nodes i1 to the point l1
+    const int f = o / (2 * NF), sd = (o / NF) % 2, r = o % NF;
+    if(sd == 1 && !interior(f)) continue;
+    const int l1 = r % NP, i2 = r / NP;
+    T va = 0, da = 0, vb = 0;
+#pragma unroll
+    for(int i1 = 0; i1 < NP; ++i1)
+    {
+      const T a = PA[f][sd][i1 + NP * i2], pb = PB[f][sd][i1 + NP * i2];
+      const T v = tl.V[l1 * NP + i1];
+      va += v * a;
+      da += tl.Dv[l1 * NP + i1] * a;
+      vb += v * pb;
+    }
+    VA[f][sd][r] = va;
+    DA[f][sd][r] = da;
+    VB[f][sd][r] = vb;
+  }
+  __syncthreads();
+
+  // ---------------- P4: z-integration; face fluxes at every face point
+  if(act)
+  {
+    if constexpr(DIM == 3)
+    {
+      const int base = ts - iz * NP * NP;
+      T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for(int q = 0; q < NP; ++q)
+      {
+        const T vz = tl.V[q * NP + iz], dz = tl.Dv[q * NP + iz];
+        s0 += vz * A0[base + q * NP * NP];
+        s1 += vz * A1[base + q * NP * NP];
+        s2 += dz * A2[base + q * NP * NP];
+      }
+      B0[t] = s0;
+      B1[t] = s1;
+      B2[t] = s2;
+    }
+  }
+  const T gamma = p == 0 ? T(1) : T(p) * T(p + 1);
+  for(int pt = t; pt < NFACE * NF; pt += NT)
+  {
+    const int f = pt / NF, l = pt % NF;
+    const int d = f / 2, side = f % 2;
+    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    const bool inner = interior(f);
+    const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
+    // traces: value, normal and tangential reference derivatives of both sides
+    T tr[2][4];  // [side][value, d_normal, d_e1, d_e2]
+#pragma unroll
+    for(int sd = 0; sd < 2; ++sd)
+    {
+      T u = 0, gn = 0, gt1 = 0, gt2 = 0;
+      if(sd == 0 || inner)
+      {
+#pragma unroll
+        for(int i2 = 0; i2 < NP2; ++i2)
+        {
+          const int r = l1 + NP * i2;
+          const T v2 = DIM == 3 ? tl.V[l2 * NP + i2] : T(1);
+          const T d2 = DIM == 3 ? tl.Dv[l2 * NP + i2] : T(0);
+          const T va = VA[f][sd][r];
+          u += v2 * va;
+          gn += v2 * VB[f][sd][r];
+          gt1 += v2 * DA[f][sd][r];
+          gt2 += d2 * va;
+        }
+      }
+      tr[sd][0] = u;
+      tr[sd][1] = gn;
+      tr[sd][2] = gt1;
+      tr[sd][3] = gt2;
+    }
+    T xi[3];
+    const T q1 = tl.qx[l1], q2 = tl.qx[l2];
+#pragma unroll
+    for(int e = 0; e < DIM; ++e) xi[e] = e == d ? T(side) : (e == e1 ? q1 : q2);
+    T J[3][3], Jio[3][3];
+    coef_jacobian<DIM>(C[0], xi, J);
+    const T deto = invert<DIM>(J, Jio);
+    // outward unit normal of this cell and surface element (Nanson)
+    T nv[3] = {0, 0, 0}, nn = 0;
+#pragma unroll
+    for(int a = 0; a < DIM; ++a)
+    {
+      nv[a] = side == 0 ? -Jio[d][a] : Jio[d][a];
       nn += nv[a] * nv[a];
     }
     nn = sqrt(nn);
-    for(int a = 0; a < DIM; ++a) nv[a] /= nn;
-    const double dS = fabs(deto) * nn * wq;
+    const T inn = T(1) / nn;
+#pragma unroll
+    for(int a = 0; a < DIM; ++a) nv[a] *= inn;
+    const T wq = tl.qw[l1] * (DIM == 3 ? tl.qw[l2] : T(1));
+    const T dS = fabs(deto) * nn * wq;
     // reference-gradient coefficients of the normal derivative: co = J^{-1} n
-    double co[3] = {0, 0, 0};
+    T co[3] = {0, 0, 0};
+#pragma unroll
     for(int bb = 0; bb < DIM; ++bb)
     {
-      double s2 = 0.0;
-      for(int a = 0; a < DIM; ++a) s2 += Jio[bb][a] * nv[a];
-      co[bb] = s2;
+      T s = 0;
+#pragma unroll
+      for(int a = 0; a < DIM; ++a) s += Jio[bb][a] * nv[a];
+      co[bb] = s;
     }
-    const double dno = gon * co[d] + got1 * co[e1] + (DIM == 3 ? got2 * co[e2] : 0.0);
-    double dnn = 0.0;
-    if(interior)
+    const T dno = tr[0][1] * co[d] + tr[0][2] * co[e1] + (DIM == 3 ? tr[0][3] * co[e2] : T(0));
+    T jv, av, fnd, cnd = T(1);
+    if(inner)
     {
-      double Jn[3][3], Jin[3][3];
-      map_jacobian<DIM>(XN[f], xin, Jn);
+      xi[d] = T(1 - side);
+      T Jn[3][3], Jin[3][3];
+      coef_jacobian<DIM>(C[1 + f], xi, Jn);
       invert<DIM>(Jn, Jin);
-      double cn[3] = {0, 0, 0};
+      T cn[3] = {0, 0, 0};
+#pragma unroll
       for(int bb = 0; bb < DIM; ++bb)
       {
-        double s2 = 0.0;
-        for(int a = 0; a < DIM; ++a) s2 += Jin[bb][a] * nv[a];
-        cn[bb] = s2;
+        T s = 0;
+#pragma unroll
+        for(int a = 0; a < DIM; ++a) s += Jin[bb][a] * nv[a];
+        cn[bb] = s;
       }
-      dnn = gnn * cn[d] + gnt1 * cn[e1] + (DIM == 3 ? gnt2 * cn[e2] : 0.0);
-    }
-    double fval, fnd;
-    if(interior)
-    {
-      const double sigma = ip * 0.5 * 2.0 * gamma * (0.5 / HP[f][0] + 0.5 / HP[f][1]);
-      const double jump = uo - un, avg = 0.5 * (dno + dnn);
-      fval = dS * (jump * sigma - avg);
-      fnd = dS * (-0.5 * jump);
+      cnd = cn[d];
+      const T dnn =
+        tr[1][1] * cn[d] + tr[1][2] * cn[e1] + (DIM == 3 ? tr[1][3] * cn[e2] : T(0));
+      const T jump = tr[0][0] - tr[1][0];
+      jv = dS * jump;
+      av = -dS * T(0.5) * (dno + dnn);
+      fnd = dS * T(-0.5) * jump;
     }
     else
     {
-      const double sigma = ip * 2.0 * gamma / HP[f][0];
-      fval = dS * (uo * sigma - dno);
-      fnd = dS * (-uo);
+      jv = dS * tr[0][0];
+      av = -dS * dno;
+      fnd = -dS * tr[0][0];
     }
-    FV[f][l] = fval;
+    JV[f][l] = jv;
+    AV[f][l] = av;
+#pragma unroll
     for(int bb = 0; bb < DIM; ++bb) FH[f][bb][l] = fnd * co[bb];
+    // penalty lengths h = 1/|(J^{-1} n)_d| at the first face quadrature point of
+    // both cells (apps/laplace_integrator.h:651-656, 692-693)
+    if(l == 0)
+    {
+      HP[f][0] = T(1) / fabs(co[d]);
+      HP[f][1] = inner ? T(1) / fabs(cnd) : T(1);
+    }
   }
   __syncthreads();
+
+  // ---------------- P5: y-integration; face projections on the tangential
+  // test functions (into PA / PB, last read in P3)
   if(act)
   {
-    // test functions of this cell: phi_i = f(i_d) * tangential factors, all faces
+    if constexpr(DIM == 3)
+    {
+      const int base = ts - iy * NP;
+      T s0 = 0, s12 = 0;
+#pragma unroll
+      for(int q = 0; q < NP; ++q)
+      {
+        const T vy = tl.V[q * NP + iy], dy = tl.Dv[q * NP + iy];
+        s0 += vy * B0[base + q * NP];
+        s12 += dy * B1[base + q * NP] + vy * B2[base + q * NP];
+      }
+      A2[t] = s0;    // A2 was last read in P4
+      U[t] = s12;    // U was last read in P2
+    }
+    else
+    {
+      const int base = ts - iy * NP;
+      T s0 = 0, s1 = 0;
+#pragma unroll
+      for(int q = 0; q < NP; ++q)
+      {
+        s0 += tl.V[q * NP + iy] * A0[base + q * NP];
+        s1 += tl.Dv[q * NP + iy] * A1[base + q * NP];
+      }
+      A2[t] = s0;
+      U[t] = s1;
+    }
+  }
+  for(int it = t; it < NFACE * NF; it += NT)
+  {
+    const int f = it / NF, m = it % NF;
+    const int d = f / 2;
+    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+    const T sigma = interior(f) ? T(ip) * gamma * (T(0.5) / HP[f][0] + T(0.5) / HP[f][1])
+                                : T(ip) * T(2) * gamma / HP[f][0];
+    T pv = 0, pd = 0;
+#pragma unroll
+    for(int l = 0; l < NF; ++l)
+    {
+      const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
+      const T t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
+      const T t2v = DIM == 3 ? tl.V[l2 * NP + i2] : T(1);
+      const T t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : T(0);
+      const T wvv = t1v * t2v;
+      pv += wvv * (sigma * JV[f][l] + AV[f][l]) + t1d * t2v * FH[f][e1][l];
+      if(DIM == 3) pv += t1v * t2d * FH[f][e2][l];
+      pd += wvv * FH[f][d][l];
+    }
+    PA[f][0][m] = pv;
+    PB[f][0][m] = pd;
+  }
+  __syncthreads();
+
+  // ---------------- P6: x-integration and the face terms of this test function
+  if(act)
+  {
+    const int base = ts - ix;
+    T acc = 0;
+#pragma unroll
+    for(int q = 0; q < NP; ++q)
+      acc += tl.Dv[q * NP + ix] * A2[base + q] + tl.V[q * NP + ix] * U[base + q];
+    const int ti[3] = {ix, iy, iz};
+#pragma unroll
     for(int f = 0; f < NFACE; ++f)
     {
       const int d = f / 2, side = f % 2;
       const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
-      const int id = tid[d], i1 = tid[e1], i2 = DIM == 3 ? tid[e2] : 0;
-      const double fv = tl.f[side][id], fdv = tl.fd[side][id];
-      double s = 0.0;
-      for(int l = 0; l < NF; ++l)
-      {
-        const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
-        const double t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
-        const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
-        const double t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : 0.0;
-        s += fv * t1v * t2v * FV[f][l];
-        s += fdv * t1v * t2v * FH[f][d][l];
-        s += fv * t1d * t2v * FH[f][e1][l];
-        if(DIM == 3) s += fv * t1v * t2d * FH[f][e2][l];
-      }
-      acc += s;
+      const int m = ti[e1] + (DIM == 3 ? NP * ti[e2] : 0);
+      acc += tl.f[side][ti[d]] * PA[f][0][m] + tl.fd[side][ti[d]] * PB[f][0][m];
     }
-  }
-  if(act)
-  {
     if(RESIDUAL)
-      y[cell * NDC + t] = VT(double(b[cell * NDC + t]) - acc);
+      y[cell * NDC + t] = b[cell * NDC + t] - acc;
     else
-      y[cell * NDC + t] = VT(acc);
+      y[cell * NDC + t] = acc;
   }
 }
 
