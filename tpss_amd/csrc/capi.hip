@@ -47,32 +47,32 @@ constexpr int cells_per_block()
 }
 
 // y = A x (or y = b - A x), optionally only on a list of cells; vectors of type VT
-template<int NP>
-ElastTab<NP> make_elast_tab(const tpss_level *lv)
+template<int NP, typename T>
+ElastTab<NP, T> make_elast_tab(const tpss_level *lv)
 {
-  ElastTab<NP> tab;
+  ElastTab<NP, T> tab;
   const Basis1D bs = make_basis(NP - 1);
   const double h = lv->h;
   for(int q = 0; q < NP; ++q)
   {
-    tab.w[q] = h * bs.qw[q];
+    tab.w[q] = T(h * bs.qw[q]);
     for(int i = 0; i < NP; ++i)
     {
-      tab.V[q * NP + i] = bs.val[i * NP + q];
-      tab.G[q * NP + i] = bs.grad[i * NP + q] / h;
+      tab.V[q * NP + i] = T(bs.val[i * NP + q]);
+      tab.G[q * NP + i] = T(bs.grad[i * NP + q] / h);
     }
   }
   for(int f = 0; f < 2; ++f)
     for(int i = 0; i < NP; ++i)
     {
-      tab.f[f][i] = bs.fval[i * 2 + f];
-      tab.fd[f][i] = bs.fgrad[i * 2 + f] / h;
+      tab.f[f][i] = T(bs.fval[i * 2 + f]);
+      tab.fd[f][i] = T(bs.fgrad[i * 2 + f] / h);
     }
   const int p = NP - 1;
   const double gamma = p == 0 ? 1.0 : double(p) * (p + 1);
-  tab.mu = lv->mu;
-  tab.lambda = lv->lambda;
-  tab.sigma = lv->ip * gamma / h;  // Laplace::compute_penalty of two cells of size h
+  tab.mu = T(lv->mu);
+  tab.lambda = T(lv->lambda);
+  tab.sigma = T(lv->ip * gamma / h);  // Laplace::compute_penalty of two cells of size h
   return tab;
 }
 
@@ -85,7 +85,7 @@ void operator_apply(const tpss_level *lv, VT *y, const VT *x, const VT *b, const
     dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
       constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
       constexpr int NDC = cpow(NP, DIM);
-      const auto tab = make_elast_tab<NP>(lv);
+      const auto tab = make_elast_tab<NP, VT>(lv);
       const int64_t count = cells ? n_list : lv->ncell;
       if(count == 0) return;
       (void)NDC;

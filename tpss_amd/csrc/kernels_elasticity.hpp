@@ -11,12 +11,16 @@
 //
 // Vectors are component-blocked: x[c * Ns + cell * NDC + local].  CPB cells per
 // block (64 / NDC for small cells), one thread per cell DoF / quadrature point
-// (QGauss(p+1)).  Faces are evaluated element-centrically (every output entry
-// is written by exactly one thread): per face the traces of both sides are
-// contracted along the normal for all tangential nodes at once, interpolated
-// tangentially per face quadrature point, and the fluxes integrated back.
-// The face direction is a compile-time constant, so no per-thread array is
-// indexed at run time.  Arithmetic in FP64.
+// (QGauss(p+1)) handling all components; arithmetic in the vector type.
+//   cell term: the gradients of all components by sum factorisation (x, y, z
+//     stages sharing their 1D interpolations), the stress times JxW, and its
+//     integration against the test gradients (z, y, x stages);
+//   faces, one direction at a time (both faces of the cell): traces of both
+//     sides contracted along the normal (the neighbour's straight from global
+//     memory), fluxes per face point, fluxes projected on the tangential test
+//     functions, and every thread adds the terms of its test function.
+// Faces are evaluated element-centrically (every output entry is written by
+// exactly one thread).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,34 +29,18 @@
 #include <type_traits>
 
 #include "kernels.hpp"
-#include "kernels_general.hpp"
 
 namespace tpss
 {
-// contraction of an LDS tensor along direction e: out(t) = sum_j M[o*NP + j] src(t|e->j),
-// o = t_e; with TRANS the matrix is read as M[j*NP + o]
-template<int DIM, int NP, bool TRANS>
-__device__ __forceinline__ double contract(const double *src, const double *M, const int *tid,
-                                           const int t, const int e)
-{
-  constexpr int ST[3] = {1, NP, NP * NP};
-  const int o = tid[e];
-  const int base = t - o * ST[e];
-  double s = 0.0;
-#pragma unroll
-  for(int j = 0; j < NP; ++j) s += (TRANS ? M[j * NP + o] : M[o * NP + j]) * src[base + j * ST[e]];
-  return s;
-}
-
-template<int NP>
+template<int NP, typename T>
 struct ElastTab
 {
-  double V[NP * NP];   // V[q*NP + i] = phi_i(x_q)
-  double G[NP * NP];   // G[q*NP + i] = phi_i'(x_q) / h
-  double f[2][NP];     // phi_i(0), phi_i(1)
-  double fd[2][NP];    // phi_i'(0) / h, phi_i'(1) / h
-  double w[NP];        // h * Gauss weight
-  double mu, lambda, sigma;
+  T V[NP * NP];   // V[q*NP + i] = phi_i(x_q)
+  T G[NP * NP];   // G[q*NP + i] = phi_i'(x_q) / h
+  T f[2][NP];     // phi_i(0), phi_i(1)
+  T fd[2][NP];    // phi_i'(0) / h, phi_i'(1) / h
+  T w[NP];        // h * Gauss weight
+  T mu, lambda, sigma;
 };
 
 template<int DIM, int NP>
@@ -68,21 +56,25 @@ constexpr int elasticity_threads()
 }
 
 template<int DIM, int NP, typename VT, bool RESIDUAL>
-__global__ void __launch_bounds__((elasticity_threads<DIM, NP>()), 4)
+__global__ void __launch_bounds__((elasticity_threads<DIM, NP>()))
 k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
-             const ElastTab<NP> tab, const int n, const int64_t Ns,
+             const ElastTab<NP, VT> tab, const int n, const int64_t Ns,
              const uint32_t *__restrict__ cells, const int64_t count)
 {
+  using T = VT;
   constexpr int NDC = cpow(NP, DIM);
   constexpr int NF = cpow(NP, DIM - 1);
   constexpr int CPB = elasticity_cells_per_block<DIM, NP>();
+  constexpr int NT = elasticity_threads<DIM, NP>();
   constexpr int ST[3] = {1, NP, NP * NP};
-  __shared__ double U[CPB][DIM][NDC], UN[CPB][DIM][NDC];
-  __shared__ double A[CPB][NDC], B[CPB][NDC];
-  __shared__ double S[CPB][DIM][DIM][NDC];
-  __shared__ double PA[CPB][2][DIM][NF], PB[CPB][2][DIM][NF];
-  __shared__ double FA[CPB][DIM][NF], FB[CPB][DIM][DIM][NF];
-  __shared__ ElastTab<NP> tl;
+  constexpr int NS = DIM * DIM;  // stage arrays per cell
+  __shared__ T U[CPB][DIM][NDC];
+  __shared__ T R1[CPB][NS][NDC], R2[CPB][NS][NDC];  // stage arrays, used in turn
+  // normal traces [face side][cell side][component][tangential node]; later the
+  // projections of the fluxes on the tangential test functions ([.][0][.][.])
+  __shared__ T PA[CPB][2][2][DIM][NF], PB[CPB][2][2][DIM][NF];
+  __shared__ T FA[CPB][2][DIM][NF], FB[CPB][2][DIM][DIM][NF];
+  __shared__ ElastTab<NP, T> tl;
 
   const int t = threadIdx.x;
   const int cs = t / NDC, ts0 = t % NDC;
@@ -90,234 +82,386 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
   const bool act = cs < CPB && slot < count;
   const int c0 = cs < CPB ? cs : 0;  // LDS slot of this thread (idle threads alias slot 0)
   const int ts = act ? ts0 : 0;
-  const int64_t cell = act ? (cells ? int64_t(cells[slot]) : slot) : 0;
-  int cc[3] = {0, 0, 0};
+  auto cell_of = [&](int s) -> int64_t {
+    const int64_t sl = int64_t(blockIdx.x) * CPB + s;
+    return sl < count ? (cells ? int64_t(cells[sl]) : sl) : int64_t(-1);
+  };
+  const int64_t cell = act ? cell_of(cs) : 0;
+  const int ix = ts % NP, iy = (ts / NP) % NP, iz = DIM == 3 ? ts / (NP * NP) : 0;
+  const int ti[3] = {ix, iy, iz};
+
   {
-    int64_t r = cell;
-    for(int d = 0; d < DIM; ++d)
-    {
-      cc[d] = int(r % n);
-      r /= n;
-    }
+    const T *src = reinterpret_cast<const T *>(&tab);
+    T *dst = reinterpret_cast<T *>(&tl);
+    constexpr int NTAB = int(sizeof(ElastTab<NP, T>) / sizeof(T));
+    for(int i = t; i < NTAB; i += NT) dst[i] = src[i];
   }
-  int tid[3] = {0, 0, 0};
-  {
-    int r = ts;
-    for(int d = 0; d < DIM; ++d)
-    {
-      tid[d] = r % NP;
-      r /= NP;
-    }
-  }
-  if(t == 0) tl = tab;
   if(act)
-    for(int c = 0; c < DIM; ++c) U[c0][c][ts] = double(x[c * Ns + cell * NDC + ts]);
+#pragma unroll
+    for(int c = 0; c < DIM; ++c) U[c0][c][ts] = x[c * Ns + cell * NDC + ts];
   __syncthreads();
 
-  // ---------------- cell term: physical gradients at the Gauss points
-  for(int c = 0; c < DIM; ++c)
-    for(int a = 0; a < DIM; ++a)
-    {
-      const double *src = U[c0][c];
+  // ---------------- cell term
+  T acc[DIM];
 #pragma unroll
-      for(int e = 0; e < DIM; ++e)
-      {
-        const double v = contract<DIM, NP, false>(src, e == a ? tl.G : tl.V, tid, ts, e);
-        double *dst = (e % 2 == 0) ? A[c0] : B[c0];
-        __syncthreads();
-        if(act) dst[ts] = v;
-        __syncthreads();
-        src = dst;
-      }
-      const double g = src[ts];
-      __syncthreads();
-      if(act) S[c0][c][a][ts] = g;
-    }
-  __syncthreads();
-  // stress times JxW: mu (grad u + grad u^T) + lambda div u I
+  for(int c = 0; c < DIM; ++c) acc[c] = T(0);
   {
-    double g[DIM][DIM], w = 1.0;
-    for(int e = 0; e < DIM; ++e) w *= tl.w[tid[e]];
-    double div = 0.0;
-#pragma unroll
-    for(int c = 0; c < DIM; ++c)
-    {
-#pragma unroll
-      for(int a = 0; a < DIM; ++a) g[c][a] = S[c0][c][a][ts];
-      div += g[c][c];
-    }
-    __syncthreads();
+    // x-stage: R1[2c] = V_x u_c, R1[2c+1] = G_x u_c
     if(act)
+    {
+      const int base = ts - ix;
 #pragma unroll
       for(int c = 0; c < DIM; ++c)
-#pragma unroll
-        for(int a = 0; a < DIM; ++a)
-          S[c0][c][a][ts] = w * (tl.mu * (g[c][a] + g[a][c]) + (c == a ? tl.lambda * div : 0.0));
-  }
-  __syncthreads();
-  double acc[DIM];
-#pragma unroll
-  for(int c = 0; c < DIM; ++c)
-  {
-    acc[c] = 0.0;
-    for(int a = 0; a < DIM; ++a)
-    {
-      const double *src = S[c0][c][a];
-#pragma unroll
-      for(int e = 0; e < DIM; ++e)
       {
-        const double v = contract<DIM, NP, true>(src, e == a ? tl.G : tl.V, tid, ts, e);
-        double *dst = (e % 2 == 0) ? A[c0] : B[c0];
-        __syncthreads();
-        if(act) dst[ts] = v;
-        __syncthreads();
-        src = dst;
+        T v = 0, g = 0;
+#pragma unroll
+        for(int j = 0; j < NP; ++j)
+        {
+          const T u = U[c0][c][base + j];
+          v += tl.V[ix * NP + j] * u;
+          g += tl.G[ix * NP + j] * u;
+        }
+        R1[c0][2 * c][ts] = v;
+        R1[c0][2 * c + 1][ts] = g;
       }
-      acc[c] += src[ts];
+    }
+    __syncthreads();
+    T g[DIM][DIM];  // g[c][a] = d u_c / d x_a at the quadrature point ts
+    if constexpr(DIM == 3)
+    {
+      // y-stage: R2[3c] = V_y G_x u_c, R2[3c+1] = G_y V_x u_c, R2[3c+2] = V_y V_x u_c
+      if(act)
+      {
+        const int base = ts - iy * NP;
+#pragma unroll
+        for(int c = 0; c < DIM; ++c)
+        {
+          T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+          for(int j = 0; j < NP; ++j)
+          {
+            const T av = R1[c0][2 * c][base + j * NP], ag = R1[c0][2 * c + 1][base + j * NP];
+            const T vy = tl.V[iy * NP + j];
+            s0 += vy * ag;
+            s1 += tl.G[iy * NP + j] * av;
+            s2 += vy * av;
+          }
+          R2[c0][3 * c][ts] = s0;
+          R2[c0][3 * c + 1][ts] = s1;
+          R2[c0][3 * c + 2][ts] = s2;
+        }
+      }
       __syncthreads();
+      // z-stage into registers
+      const int base = ts - iz * NP * NP;
+#pragma unroll
+      for(int c = 0; c < DIM; ++c)
+      {
+        T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for(int j = 0; j < NP; ++j)
+        {
+          const T vz = tl.V[iz * NP + j];
+          s0 += vz * R2[c0][3 * c][base + j * NP * NP];
+          s1 += vz * R2[c0][3 * c + 1][base + j * NP * NP];
+          s2 += tl.G[iz * NP + j] * R2[c0][3 * c + 2][base + j * NP * NP];
+        }
+        g[c][0] = s0;
+        g[c][1] = s1;
+        g[c][2] = s2;
+      }
+    }
+    else
+    {
+      const int base = ts - iy * NP;
+#pragma unroll
+      for(int c = 0; c < DIM; ++c)
+      {
+        T s0 = 0, s1 = 0;
+#pragma unroll
+        for(int j = 0; j < NP; ++j)
+        {
+          s0 += tl.V[iy * NP + j] * R1[c0][2 * c + 1][base + j * NP];
+          s1 += tl.G[iy * NP + j] * R1[c0][2 * c][base + j * NP];
+        }
+        g[c][0] = s0;
+        g[c][1] = s1;
+      }
+    }
+    // stress times JxW: mu (grad u + grad u^T) + lambda div u I, into the array
+    // not read in the last stage (3D: R1, 2D: R2)
+    {
+      T w = T(1), div = T(0);
+#pragma unroll
+      for(int e = 0; e < DIM; ++e) w *= tl.w[ti[e]];
+#pragma unroll
+      for(int c = 0; c < DIM; ++c) div += g[c][c];
+      T(*SG)[NDC] = DIM == 3 ? R1[c0] : R2[c0];
+      if(act)
+#pragma unroll
+        for(int c = 0; c < DIM; ++c)
+#pragma unroll
+          for(int a = 0; a < DIM; ++a)
+            SG[c * DIM + a][ts] =
+              w * (tl.mu * (g[c][a] + g[a][c]) + (c == a ? tl.lambda * div : T(0)));
+    }
+    __syncthreads();
+    if constexpr(DIM == 3)
+    {
+      // z-integration: R2[3c] = V_z^T s_c0, R2[3c+1] = V_z^T s_c1, R2[3c+2] = G_z^T s_c2
+      if(act)
+      {
+        const int base = ts - iz * NP * NP;
+#pragma unroll
+        for(int c = 0; c < DIM; ++c)
+        {
+          T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+          for(int q = 0; q < NP; ++q)
+          {
+            const T vz = tl.V[q * NP + iz];
+            s0 += vz * R1[c0][c * 3][base + q * NP * NP];
+            s1 += vz * R1[c0][c * 3 + 1][base + q * NP * NP];
+            s2 += tl.G[q * NP + iz] * R1[c0][c * 3 + 2][base + q * NP * NP];
+          }
+          R2[c0][3 * c][ts] = s0;
+          R2[c0][3 * c + 1][ts] = s1;
+          R2[c0][3 * c + 2][ts] = s2;
+        }
+      }
+      __syncthreads();
+      // y-integration: R1[2c] = V_y^T (.)0, R1[2c+1] = G_y^T (.)1 + V_y^T (.)2
+      if(act)
+      {
+        const int base = ts - iy * NP;
+#pragma unroll
+        for(int c = 0; c < DIM; ++c)
+        {
+          T s0 = 0, s1 = 0;
+#pragma unroll
+          for(int q = 0; q < NP; ++q)
+          {
+            const T vy = tl.V[q * NP + iy];
+            s0 += vy * R2[c0][3 * c][base + q * NP];
+            s1 += tl.G[q * NP + iy] * R2[c0][3 * c + 1][base + q * NP] +
+                  vy * R2[c0][3 * c + 2][base + q * NP];
+          }
+          R1[c0][2 * c][ts] = s0;
+          R1[c0][2 * c + 1][ts] = s1;
+        }
+      }
+    }
+    else
+    {
+      // y-integration: R1[2c] = V_y^T s_c0, R1[2c+1] = G_y^T s_c1
+      if(act)
+      {
+        const int base = ts - iy * NP;
+#pragma unroll
+        for(int c = 0; c < DIM; ++c)
+        {
+          T s0 = 0, s1 = 0;
+#pragma unroll
+          for(int q = 0; q < NP; ++q)
+          {
+            s0 += tl.V[q * NP + iy] * R2[c0][c * 2][base + q * NP];
+            s1 += tl.G[q * NP + iy] * R2[c0][c * 2 + 1][base + q * NP];
+          }
+          R1[c0][2 * c][ts] = s0;
+          R1[c0][2 * c + 1][ts] = s1;
+        }
+      }
+    }
+    __syncthreads();
+    // x-integration
+    if(act)
+    {
+      const int base = ts - ix;
+#pragma unroll
+      for(int c = 0; c < DIM; ++c)
+      {
+        T s = 0;
+#pragma unroll
+        for(int q = 0; q < NP; ++q)
+          s += tl.G[q * NP + ix] * R1[c0][2 * c][base + q] +
+               tl.V[q * NP + ix] * R1[c0][2 * c + 1][base + q];
+        acc[c] = s;
+      }
     }
   }
 
-  // ---------------- faces, direction D fixed at compile time
+  // ---------------- faces, direction D fixed at compile time (both faces)
   auto face_dir = [&](auto Dc, int64_t cstride) {
     constexpr int D = decltype(Dc)::value;
     constexpr int E1 = D == 0 ? 1 : 0;
     constexpr int E2 = DIM == 3 ? (D == 2 ? 1 : 2) : E1;
-#pragma unroll 1
-    for(int side = 0; side < 2; ++side)
+    auto coord = [&](int64_t ce) { return int((ce / cstride) % n); };
+    __syncthreads();  // PA / PB of the previous direction have been read
+    // traces along the normal: value (PA) and normal derivative (PB) of both
+    // cell sides for every component and tangential node
+    for(int o = t; o < CPB * 2 * 2 * DIM * NF; o += NT)
     {
-      const bool interior = side == 1 ? cc[D] < n - 1 : cc[D] > 0;
-      const int64_t nb = interior ? (side == 1 ? cell + cstride : cell - cstride) : cell;
-      const int nside = 1 - side;
-      if(interior && act)
-        for(int c = 0; c < DIM; ++c) UN[c0][c][ts] = double(x[c * Ns + nb * NDC + ts]);
-      __syncthreads();
-      // contraction along the normal: value (PA) and normal derivative (PB) traces
-      // of both sides for every component and tangential node
-      if(act)
-        for(int o = ts; o < 2 * DIM * NF; o += NDC)
-        {
-          const int sd = o / (DIM * NF), rem = o % (DIM * NF), c = rem / NF, m = rem % NF;
-          if(sd == 1 && !interior) continue;
-          const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-          const int base = i1 * ST[E1] + (DIM == 3 ? i2 * ST[E2] : 0);
-          const double *src = sd == 0 ? U[c0][c] : UN[c0][c];
-          const int fs = sd == 0 ? side : nside;
-          double a = 0.0, bsum = 0.0;
-#pragma unroll
-          for(int id = 0; id < NP; ++id)
-          {
-            const double u = src[base + id * ST[D]];
-            a += tl.f[fs][id] * u;
-            bsum += tl.fd[fs][id] * u;
-          }
-          PA[c0][sd][c][m] = a;
-          PB[c0][sd][c][m] = bsum;
-        }
-      __syncthreads();
-      if(act && ts < NF)
+      const int s = o / (2 * 2 * DIM * NF), r = o % (2 * 2 * DIM * NF);
+      const int side = r / (2 * DIM * NF), sd = (r / (DIM * NF)) % 2;
+      const int c = (r / NF) % DIM, m = r % NF;
+      const int64_t ce = cell_of(s);
+      if(ce < 0) continue;
+      const int cd = coord(ce);
+      const bool inner = side == 1 ? cd < n - 1 : cd > 0;
+      if(sd == 1 && !inner) continue;
+      const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+      const int base = i1 * ST[E1] + (DIM == 3 ? i2 * ST[E2] : 0);
+      const int fs = sd == 0 ? side : 1 - side;
+      T a = 0, bs = 0;
+      if(sd == 0)
       {
-        const int l1 = ts % NP, l2 = DIM == 3 ? ts / NP : 0;
-        // traces and physical gradients of this cell (o) and the exterior state (x)
-        double uo[DIM], go[DIM][DIM], ux[DIM], gx[DIM][DIM];
 #pragma unroll
-        for(int c = 0; c < DIM; ++c)
+        for(int id = 0; id < NP; ++id)
         {
-          uo[c] = ux[c] = 0.0;
-#pragma unroll
-          for(int a = 0; a < DIM; ++a) go[c][a] = gx[c][a] = 0.0;
-        }
-#pragma unroll 1
-        for(int m = 0; m < NF; ++m)
-        {
-          const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-          const double t1v = tl.V[l1 * NP + i1], t1d = tl.G[l1 * NP + i1];
-          const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
-          const double t2d = DIM == 3 ? tl.G[l2 * NP + i2] : 0.0;
-          const double wvv = t1v * t2v, wdv = t1d * t2v, wvd = t1v * t2d;
-#pragma unroll
-          for(int c = 0; c < DIM; ++c)
-          {
-            const double a = PA[c0][0][c][m];
-            uo[c] += wvv * a;
-            go[c][D] += wvv * PB[c0][0][c][m];
-            go[c][E1] += wdv * a;
-            if(DIM == 3) go[c][E2] += wvd * a;
-            if(interior)
-            {
-              const double a1 = PA[c0][1][c][m];
-              ux[c] += wvv * a1;
-              gx[c][D] += wvv * PB[c0][1][c][m];
-              gx[c][E1] += wdv * a1;
-              if(DIM == 3) gx[c][E2] += wvd * a1;
-            }
-          }
-        }
-        if(!interior)
-#pragma unroll
-          for(int c = 0; c < DIM; ++c)
-          {
-            ux[c] = -uo[c];
-#pragma unroll
-            for(int a = 0; a < DIM; ++a) gx[c][a] = go[c][a];
-          }
-        const double nrm = side == 1 ? 1.0 : -1.0;
-        const double dS = tl.w[l1] * (DIM == 3 ? tl.w[l2] : 1.0);
-        double J[DIM], divo = 0.0, divx = 0.0;
-#pragma unroll
-        for(int c = 0; c < DIM; ++c)
-        {
-          J[c] = uo[c] - ux[c];
-          divo += go[c][c];
-          divx += gx[c][c];
-        }
-#pragma unroll
-        for(int c = 0; c < DIM; ++c)
-        {
-          // {2 mu eps(u) n + lambda div(u) n}_c
-          const double fo = tl.mu * (go[c][D] + go[D][c]) + (c == D ? tl.lambda * divo : 0.0);
-          const double fx = tl.mu * (gx[c][D] + gx[D][c]) + (c == D ? tl.lambda * divx : 0.0);
-          const double F = 0.5 * (fo + fx) * nrm;
-          FA[c0][c][ts] = dS * (-F + 2.0 * tl.mu * tl.sigma * J[c] +
-                                (c == D ? tl.lambda * tl.sigma * J[D] : 0.0));
-          // -1/2 (2 mu eps(v) n . [u] + lambda div v [u].n)
-          double fb[DIM];
-#pragma unroll
-          for(int a = 0; a < DIM; ++a) fb[a] = 0.0;
-          fb[D] += -0.5 * tl.mu * nrm * J[c];
-          if(c == D)
-#pragma unroll
-            for(int a = 0; a < DIM; ++a) fb[a] += -0.5 * tl.mu * nrm * J[a];
-          fb[c] += -0.5 * tl.lambda * nrm * J[D];
-#pragma unroll
-          for(int a = 0; a < DIM; ++a) FB[c0][c][a][ts] = dS * fb[a];
+          const T u = U[s][c][base + id * ST[D]];
+          a += tl.f[fs][id] * u;
+          bs += tl.fd[fs][id] * u;
         }
       }
-      __syncthreads();
-      if(act)
+      else
       {
-        // test functions of this cell: phi_i = f(i_D) * tangential factors
-        const int id = tid[D], i1 = tid[E1], i2 = DIM == 3 ? tid[E2] : 0;
-        const double fv = tl.f[side][id], fdv = tl.fd[side][id];
-#pragma unroll 1
-        for(int l = 0; l < NF; ++l)
-        {
-          const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
-          const double t1v = tl.V[l1 * NP + i1], t1d = tl.G[l1 * NP + i1];
-          const double t2v = DIM == 3 ? tl.V[l2 * NP + i2] : 1.0;
-          const double t2d = DIM == 3 ? tl.G[l2 * NP + i2] : 0.0;
-          const double wv = fv * t1v * t2v, wn = fdv * t1v * t2v, w1 = fv * t1d * t2v,
-                       w2 = fv * t1v * t2d;
+        const int64_t nb = side == 1 ? ce + cstride : ce - cstride;
+        const VT *src = x + c * Ns + nb * NDC + base;
 #pragma unroll
-          for(int c = 0; c < DIM; ++c)
+        for(int id = 0; id < NP; ++id)
+        {
+          const T u = src[id * ST[D]];
+          a += tl.f[fs][id] * u;
+          bs += tl.fd[fs][id] * u;
+        }
+      }
+      PA[s][side][sd][c][m] = a;
+      PB[s][side][sd][c][m] = bs;
+    }
+    __syncthreads();
+    // fluxes at every face point
+    for(int o = t; o < CPB * 2 * NF; o += NT)
+    {
+      const int s = o / (2 * NF), side = (o / NF) % 2, l = o % NF;
+      const int64_t ce = cell_of(s);
+      if(ce < 0) continue;
+      const int cd = coord(ce);
+      const bool inner = side == 1 ? cd < n - 1 : cd > 0;
+      const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
+      // traces and physical gradients of this cell (o) and the exterior state (x)
+      T uo[DIM], go[DIM][DIM], ux[DIM], gx[DIM][DIM];
+#pragma unroll
+      for(int c = 0; c < DIM; ++c)
+      {
+        uo[c] = ux[c] = T(0);
+#pragma unroll
+        for(int a = 0; a < DIM; ++a) go[c][a] = gx[c][a] = T(0);
+      }
+#pragma unroll
+      for(int m = 0; m < NF; ++m)
+      {
+        const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+        const T t1v = tl.V[l1 * NP + i1], t1d = tl.G[l1 * NP + i1];
+        const T t2v = DIM == 3 ? tl.V[l2 * NP + i2] : T(1);
+        const T t2d = DIM == 3 ? tl.G[l2 * NP + i2] : T(0);
+        const T wvv = t1v * t2v, wdv = t1d * t2v, wvd = t1v * t2d;
+#pragma unroll
+        for(int c = 0; c < DIM; ++c)
+        {
+          const T a = PA[s][side][0][c][m];
+          uo[c] += wvv * a;
+          go[c][D] += wvv * PB[s][side][0][c][m];
+          go[c][E1] += wdv * a;
+          if(DIM == 3) go[c][E2] += wvd * a;
+          if(inner)
           {
-            double s = wv * FA[c0][c][l] + wn * FB[c0][c][D][l] + w1 * FB[c0][c][E1][l];
-            if(DIM == 3) s += w2 * FB[c0][c][E2][l];
-            acc[c] += s;
+            const T a1 = PA[s][side][1][c][m];
+            ux[c] += wvv * a1;
+            gx[c][D] += wvv * PB[s][side][1][c][m];
+            gx[c][E1] += wdv * a1;
+            if(DIM == 3) gx[c][E2] += wvd * a1;
           }
         }
       }
-      __syncthreads();
+      if(!inner)
+#pragma unroll
+        for(int c = 0; c < DIM; ++c)
+        {
+          ux[c] = -uo[c];
+#pragma unroll
+          for(int a = 0; a < DIM; ++a) gx[c][a] = go[c][a];
+        }
+      const T nrm = side == 1 ? T(1) : T(-1);
+      const T dS = tl.w[l1] * (DIM == 3 ? tl.w[l2] : T(1));
+      T J[DIM], divo = 0, divx = 0;
+#pragma unroll
+      for(int c = 0; c < DIM; ++c)
+      {
+        J[c] = uo[c] - ux[c];
+        divo += go[c][c];
+        divx += gx[c][c];
+      }
+      const T hm = T(-0.5) * tl.mu * nrm, hl = T(-0.5) * tl.lambda * nrm;
+#pragma unroll
+      for(int c = 0; c < DIM; ++c)
+      {
+        // {2 mu eps(u) n + lambda div(u) n}_c
+        const T fo = tl.mu * (go[c][D] + go[D][c]) + (c == D ? tl.lambda * divo : T(0));
+        const T fx = tl.mu * (gx[c][D] + gx[D][c]) + (c == D ? tl.lambda * divx : T(0));
+        const T F = T(0.5) * (fo + fx) * nrm;
+        FA[s][side][c][l] = dS * (-F + T(2) * tl.mu * tl.sigma * J[c] +
+                                  (c == D ? tl.lambda * tl.sigma * J[D] : T(0)));
+        // -1/2 (2 mu eps(v) n . [u] + lambda div v [u].n)
+        T fb[DIM];
+#pragma unroll
+        for(int a = 0; a < DIM; ++a) fb[a] = T(0);
+        fb[D] += hm * J[c];
+        if(c == D)
+#pragma unroll
+          for(int a = 0; a < DIM; ++a) fb[a] += hm * J[a];
+        fb[c] += hl * J[D];
+#pragma unroll
+        for(int a = 0; a < DIM; ++a) FB[s][side][c][a][l] = dS * fb[a];
+      }
+    }
+    __syncthreads();
+    // projections on the tangential test functions (into PA / PB [.][0])
+    for(int o = t; o < CPB * 2 * DIM * NF; o += NT)
+    {
+      const int s = o / (2 * DIM * NF), side = (o / (DIM * NF)) % 2;
+      const int c = (o / NF) % DIM, m = o % NF;
+      if(cell_of(s) < 0) continue;
+      const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+      T pv = 0, pd = 0;
+#pragma unroll
+      for(int l = 0; l < NF; ++l)
+      {
+        const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
+        const T t1v = tl.V[l1 * NP + i1], t1d = tl.G[l1 * NP + i1];
+        const T t2v = DIM == 3 ? tl.V[l2 * NP + i2] : T(1);
+        const T t2d = DIM == 3 ? tl.G[l2 * NP + i2] : T(0);
+        const T wvv = t1v * t2v;
+        pv += wvv * FA[s][side][c][l] + t1d * t2v * FB[s][side][c][E1][l];
+        if(DIM == 3) pv += t1v * t2d * FB[s][side][c][E2][l];
+        pd += wvv * FB[s][side][c][D][l];
+      }
+      PA[s][side][0][c][m] = pv;
+      PB[s][side][0][c][m] = pd;
+    }
+    __syncthreads();
+    if(act)
+    {
+      // test functions of this cell: phi_i = f(i_D) * tangential factors
+      const int id = ti[D];
+      const int m = ti[E1] + (DIM == 3 ? NP * ti[E2] : 0);
+#pragma unroll
+      for(int side = 0; side < 2; ++side)
+      {
+        const T fv = tl.f[side][id], fdv = tl.fd[side][id];
+#pragma unroll
+        for(int c = 0; c < DIM; ++c)
+          acc[c] += fv * PA[c0][side][0][c][m] + fdv * PB[c0][side][0][c][m];
+      }
     }
   };
   const int64_t nn = int64_t(n) * n;
@@ -331,9 +475,9 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
     {
       const int64_t i = c * Ns + cell * NDC + ts;
       if(RESIDUAL)
-        y[i] = VT(double(b[i]) - acc[c]);
+        y[i] = b[i] - acc[c];
       else
-        y[i] = VT(acc[c]);
+        y[i] = acc[c];
     }
 }
 
