@@ -412,17 +412,18 @@ int tpss_transfer_prolongate_add(tpss_level *coarse, tpss_level *fine, void *dst
       for(int c = 0; c < 2; ++c)
         for(int e = 0; e < NP * NP; ++e) tab.P[c][e] = P[c * NP * NP + e];
       // one scalar DG transfer per component block (MGTransferMatrixFree per block)
-      const unsigned g = blocks_for(fine->Ns, 256);
+      const unsigned g = unsigned(coarse->ncell);
+      const unsigned nt = unsigned(TransferShape<DIM, NP>::THREADS);
       for(int b = 0; b < fine->ncomp; ++b)
       {
         if(fine->vec_prec == TPSS_F64)
-          hipLaunchKernelGGL((k_prolongate_add<DIM, NP, double>), dim3(g), dim3(256), 0, s,
+          hipLaunchKernelGGL((k_prolongate_add<DIM, NP, double>), dim3(g), dim3(nt), 0, s,
                              (double *)dst_fine + b * fine->Ns,
-                             (const double *)src_coarse + b * coarse->Ns, tab, coarse->n, fine->Ns);
+                             (const double *)src_coarse + b * coarse->Ns, tab, coarse->n);
         else
-          hipLaunchKernelGGL((k_prolongate_add<DIM, NP, float>), dim3(g), dim3(256), 0, s,
+          hipLaunchKernelGGL((k_prolongate_add<DIM, NP, float>), dim3(g), dim3(nt), 0, s,
                              (float *)dst_fine + b * fine->Ns,
-                             (const float *)src_coarse + b * coarse->Ns, tab, coarse->n, fine->Ns);
+                             (const float *)src_coarse + b * coarse->Ns, tab, coarse->n);
         TPSS_HIP(hipGetLastError());
       }
     });
@@ -445,17 +446,18 @@ int tpss_transfer_restrict(tpss_level *coarse, tpss_level *fine, void *dst_coars
       prolongation_1d(NP - 1, P);
       for(int c = 0; c < 2; ++c)
         for(int e = 0; e < NP * NP; ++e) tab.P[c][e] = P[c * NP * NP + e];
-      const unsigned g = blocks_for(coarse->Ns, 256);
+      const unsigned g = unsigned(coarse->ncell);
+      const unsigned nt = unsigned(TransferShape<DIM, NP>::THREADS);
       for(int b = 0; b < fine->ncomp; ++b)
       {
         if(fine->vec_prec == TPSS_F64)
-          hipLaunchKernelGGL((k_restrict<DIM, NP, double>), dim3(g), dim3(256), 0, s,
+          hipLaunchKernelGGL((k_restrict<DIM, NP, double>), dim3(g), dim3(nt), 0, s,
                              (double *)dst_coarse + b * coarse->Ns,
-                             (const double *)src_fine + b * fine->Ns, tab, coarse->n, coarse->Ns);
+                             (const double *)src_fine + b * fine->Ns, tab, coarse->n);
         else
-          hipLaunchKernelGGL((k_restrict<DIM, NP, float>), dim3(g), dim3(256), 0, s,
+          hipLaunchKernelGGL((k_restrict<DIM, NP, float>), dim3(g), dim3(nt), 0, s,
                              (float *)dst_coarse + b * coarse->Ns,
-                             (const float *)src_fine + b * fine->Ns, tab, coarse->n, coarse->Ns);
+                             (const float *)src_fine + b * fine->Ns, tab, coarse->n);
         TPSS_HIP(hipGetLastError());
       }
     });

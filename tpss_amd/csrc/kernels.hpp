@@ -749,91 +749,160 @@ struct TransferTab
   double P[2][NP * NP];  // P[child][i*NP + j] = phi_j((x_i + child)/2)
 };
 
-// xf += P xc ; one thread per fine DoF
-template<int DIM, int NP, typename T>
-__global__ void __launch_bounds__(256)
-k_prolongate_add(T *__restrict__ xf, const T *__restrict__ xc, const TransferTab<NP> tab,
-                 const int nc, const int64_t Nf)
+// Grid transfer per coarse cell: the cell and its 2^DIM children are viewed as
+// the coarse tensor c[jx][jy][jz] (NP^DIM) and the fine tensor F[X][Y][Z]
+// ((2 NP)^DIM, X = child_x * NP + fine node), linked by the 1D prolongation
+// Pf[X][j] = P[X / NP][(X % NP) NP + j]:
+//   prolongation F = (Pf x Pf x Pf) c,   restriction c = (Pf^T x Pf^T x Pf^T) F,
+// one direction at a time through LDS (sum factorisation); one block per
+// coarse cell, fine values moved child by child (contiguous cell blocks).
+template<int DIM, int NP>
+struct TransferShape
 {
-  constexpr int NDC = cpow(NP, DIM);
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if(i >= Nf) return;
-  const int nf = 2 * nc;
-  int64_t fcell = i / NDC;
-  int l = int(i % NDC);
-  int64_t ccell = 0, st = 1;
-  int child[3], li[3];
+  static constexpr int K = 2 * NP;
+  static constexpr int NDC = cpow(NP, DIM);
+  static constexpr int NFT = cpow(K, DIM);
+  static constexpr int THREADS = NFT >= 256 ? 256 : ((NFT + 63) / 64) * 64;
+};
+
+// fine tensor index (X, Y, Z) of entry `local` of child `child`
+template<int DIM, int NP>
+__device__ __forceinline__ int fine_tensor_index(int child, int local)
+{
+  constexpr int K = 2 * NP;
+  int o = 0, st = 1;
+#pragma unroll
   for(int d = 0; d < DIM; ++d)
   {
-    const int fc = int(fcell % nf);
-    fcell /= nf;
-    child[d] = fc & 1;
-    ccell += int64_t(fc >> 1) * st;
-    st *= nc;
-    li[d] = l % NP;
-    l /= NP;
+    o += (((child >> d) & 1) * NP + local % NP) * st;
+    local /= NP;
+    st *= K;
   }
-  const T *x = xc + ccell * NDC;
-  double s = 0.0;
-  for(int jj = 0; jj < NDC; ++jj)
-  {
-    int t = jj;
-    double w = 1.0;
-#pragma unroll
-    for(int d = 0; d < DIM; ++d)
-    {
-      w *= tab.P[child[d]][li[d] * NP + (t % NP)];
-      t /= NP;
-    }
-    s += w * double(x[jj]);
-  }
-  xf[i] = T(double(xf[i]) + s);
+  return o;
 }
 
-// xc = P^T xf ; one thread per coarse DoF
-template<int DIM, int NP, typename T>
-__global__ void __launch_bounds__(256)
-k_restrict(T *__restrict__ xc, const T *__restrict__ xf, const TransferTab<NP> tab, const int nc,
-           const int64_t Nc)
+template<int DIM>
+__device__ __forceinline__ int64_t child_cell(int64_t ccell, int nc, int child)
 {
-  constexpr int NDC = cpow(NP, DIM);
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if(i >= Nc) return;
+  int64_t f = 0, st = 1;
   const int nf = 2 * nc;
-  int64_t ccell = i / NDC;
-  int l = int(i % NDC);
-  int C[3], lj[3];
+#pragma unroll
   for(int d = 0; d < DIM; ++d)
   {
-    C[d] = int(ccell % nc);
+    f += int64_t(2 * int(ccell % nc) + ((child >> d) & 1)) * st;
     ccell /= nc;
-    lj[d] = l % NP;
-    l /= NP;
+    st *= nf;
   }
-  double s = 0.0;
-  for(int child = 0; child < (1 << DIM); ++child)
+  return f;
+}
+
+// xf += P xc
+template<int DIM, int NP, typename T>
+__global__ void __launch_bounds__((TransferShape<DIM, NP>::THREADS))
+k_prolongate_add(T *__restrict__ xf, const T *__restrict__ xc, const TransferTab<NP> tab,
+                 const int nc)
+{
+  using TS = TransferShape<DIM, NP>;
+  constexpr int K = TS::K, NDC = TS::NDC, NFT = TS::NFT, NT = TS::THREADS;
+  __shared__ T Pf[K * NP];
+  __shared__ T S0[NFT], S1[NFT];
+  const int64_t cc = blockIdx.x;
+  const int t = threadIdx.x;
+  for(int i = t; i < K * NP; i += NT) Pf[i] = T(tab.P[(i / NP) / NP][((i / NP) % NP) * NP + i % NP]);
+  for(int i = t; i < NDC; i += NT) S0[i] = xc[cc * NDC + i];
+  __syncthreads();
+  // x: S1[X + K (jy + NP jz)] = sum_jx Pf[X][jx] c[jx][jy][jz]
+  for(int o = t; o < K * NDC / NP; o += NT)
   {
-    int64_t fcell = 0, st = 1;
-    for(int d = 0; d < DIM; ++d)
-    {
-      fcell += int64_t(2 * C[d] + ((child >> d) & 1)) * st;
-      st *= nf;
-    }
-    const T *x = xf + fcell * NDC;
-    for(int ii = 0; ii < NDC; ++ii)
-    {
-      int t = ii;
-      double w = 1.0;
+    const int X = o % K, r = o / K;
+    T s = 0;
 #pragma unroll
-      for(int d = 0; d < DIM; ++d)
-      {
-        w *= tab.P[(child >> d) & 1][(t % NP) * NP + lj[d]];
-        t /= NP;
-      }
-      s += w * double(x[ii]);
-    }
+    for(int j = 0; j < NP; ++j) s += Pf[X * NP + j] * S0[j + NP * r];
+    S1[o] = s;
   }
-  xc[i] = T(s);
+  __syncthreads();
+  const T *last = S1;
+  if constexpr(DIM == 3)
+  {
+    // y: S0[X + K (Y + K jz)] = sum_jy Pf[Y][jy] S1[X + K (jy + NP jz)]
+    for(int o = t; o < K * K * NP; o += NT)
+    {
+      const int X = o % K, Y = (o / K) % K, jz = o / (K * K);
+      T s = 0;
+#pragma unroll
+      for(int j = 0; j < NP; ++j) s += Pf[Y * NP + j] * S1[X + K * (j + NP * jz)];
+      S0[o] = s;
+    }
+    __syncthreads();
+    last = S0;
+  }
+  // last direction, straight into the fine vector (child by child)
+  for(int o = t; o < NFT; o += NT)
+  {
+    const int child = o / NDC, local = o % NDC;
+    const int ft = fine_tensor_index<DIM, NP>(child, local);
+    const int Z = ft / cpow(K, DIM - 1), rest = ft % cpow(K, DIM - 1);
+    T s = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j) s += Pf[Z * NP + j] * last[rest + cpow(K, DIM - 1) * j];
+    T *dst = xf + child_cell<DIM>(cc, nc, child) * NDC + local;
+    *dst = *dst + s;
+  }
+}
+
+// xc = P^T xf
+template<int DIM, int NP, typename T>
+__global__ void __launch_bounds__((TransferShape<DIM, NP>::THREADS))
+k_restrict(T *__restrict__ xc, const T *__restrict__ xf, const TransferTab<NP> tab, const int nc)
+{
+  using TS = TransferShape<DIM, NP>;
+  constexpr int K = TS::K, NDC = TS::NDC, NFT = TS::NFT, NT = TS::THREADS;
+  constexpr int KL = cpow(K, DIM - 1);
+  __shared__ T Pf[K * NP];
+  __shared__ T S0[NFT], S1[NFT];
+  const int64_t cc = blockIdx.x;
+  const int t = threadIdx.x;
+  for(int i = t; i < K * NP; i += NT) Pf[i] = T(tab.P[(i / NP) / NP][((i / NP) % NP) * NP + i % NP]);
+  for(int o = t; o < NFT; o += NT)
+  {
+    const int child = o / NDC, local = o % NDC;
+    S0[fine_tensor_index<DIM, NP>(child, local)] = xf[child_cell<DIM>(cc, nc, child) * NDC + local];
+  }
+  __syncthreads();
+  // last direction: S1[rest + KL jl] = sum_L Pf[L][jl] S0[rest + KL L]
+  for(int o = t; o < KL * NP; o += NT)
+  {
+    const int rest = o % KL, jl = o / KL;
+    T s = 0;
+#pragma unroll
+    for(int L = 0; L < K; ++L) s += Pf[L * NP + jl] * S0[rest + KL * L];
+    S1[o] = s;
+  }
+  __syncthreads();
+  const T *cur = S1;
+  if constexpr(DIM == 3)
+  {
+    // y: S0[X + K (jy + NP jz)] = sum_Y Pf[Y][jy] S1[X + K (Y + K jz)]
+    for(int o = t; o < K * NP * NP; o += NT)
+    {
+      const int X = o % K, jy = (o / K) % NP, jz = o / (K * NP);
+      T s = 0;
+#pragma unroll
+      for(int Y = 0; Y < K; ++Y) s += Pf[Y * NP + jy] * S1[X + K * (Y + K * jz)];
+      S0[o] = s;
+    }
+    __syncthreads();
+    cur = S0;
+  }
+  // x: c[jx + NP r] = sum_X Pf[X][jx] cur[X + K r]
+  for(int o = t; o < NDC; o += NT)
+  {
+    const int jx = o % NP, r = o / NP;
+    T s = 0;
+#pragma unroll
+    for(int X = 0; X < K; ++X) s += Pf[X * NP + jx] * cur[X + K * r];
+    xc[cc * NDC + o] = s;
+  }
 }
 
 // y = A x with a dense row-major FP64 matrix (coarse-grid inverse); one
