@@ -161,6 +161,19 @@ void operator_apply(const tpss_level *lv, VT *y, const VT *x, const VT *b, const
       TPSS_HIP(hipGetLastError());
       return;
     }
+    if constexpr(DIM == 3)
+    {
+      constexpr int CPB = lines_cells_per_block<NP>();
+      const unsigned grid = blocks_for(lv->ncell, CPB);
+      if(b)
+        hipLaunchKernelGGL((k_sipg_lines<NP, VT, CT, true>), dim3(grid), dim3(256), 0, s, x, y, b,
+                           tab, lv->n, lv->ncell);
+      else
+        hipLaunchKernelGGL((k_sipg_lines<NP, VT, CT, false>), dim3(grid), dim3(256), 0, s, x, y,
+                           b, tab, lv->n, lv->ncell);
+      TPSS_HIP(hipGetLastError());
+      return;
+    }
     constexpr int TX = row_tile<DIM, NP>();
     const int64_t rows = DIM == 3 ? int64_t(lv->n) * lv->n : lv->n;
     const unsigned grid = unsigned(rows * ((lv->n + TX - 1) / TX));

@@ -35,6 +35,70 @@ struct OpTab
   T Fmp[NP * NP];
 };
 
+// Contiguous rows of a cell (NP values along x) moved with 16-byte vector
+// accesses; the types only claim element alignment, so any element offset is
+// valid (global memory accesses need no 16-byte alignment on CDNA).
+typedef float tpss_f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double tpss_d2u __attribute__((ext_vector_type(2), aligned(8)));
+
+template<int NP, typename VT, typename CT>
+__device__ __forceinline__ void load_row(const VT *__restrict__ p, CT *out)
+{
+  if constexpr(std::is_same<VT, float>::value && NP % 4 == 0)
+  {
+#pragma unroll
+    for(int i = 0; i < NP; i += 4)
+    {
+      const tpss_f4u v = *reinterpret_cast<const tpss_f4u *>(p + i);
+      out[i] = CT(v.x), out[i + 1] = CT(v.y), out[i + 2] = CT(v.z), out[i + 3] = CT(v.w);
+    }
+  }
+  else if constexpr(std::is_same<VT, double>::value && NP % 2 == 0)
+  {
+#pragma unroll
+    for(int i = 0; i < NP; i += 2)
+    {
+      const tpss_d2u v = *reinterpret_cast<const tpss_d2u *>(p + i);
+      out[i] = CT(v.x), out[i + 1] = CT(v.y);
+    }
+  }
+  else
+  {
+#pragma unroll
+    for(int i = 0; i < NP; ++i) out[i] = CT(p[i]);
+  }
+}
+
+template<int NP, typename VT, typename CT>
+__device__ __forceinline__ void store_row(VT *__restrict__ p, const CT *in)
+{
+  if constexpr(std::is_same<VT, float>::value && NP % 4 == 0)
+  {
+#pragma unroll
+    for(int i = 0; i < NP; i += 4)
+    {
+      tpss_f4u v;
+      v.x = float(in[i]), v.y = float(in[i + 1]), v.z = float(in[i + 2]), v.w = float(in[i + 3]);
+      *reinterpret_cast<tpss_f4u *>(p + i) = v;
+    }
+  }
+  else if constexpr(std::is_same<VT, double>::value && NP % 2 == 0)
+  {
+#pragma unroll
+    for(int i = 0; i < NP; i += 2)
+    {
+      tpss_d2u v;
+      v.x = double(in[i]), v.y = double(in[i + 1]);
+      *reinterpret_cast<tpss_d2u *>(p + i) = v;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for(int i = 0; i < NP; ++i) p[i] = VT(in[i]);
+  }
+}
+
 template<int DIM, int NP, typename VT, typename CT, int CPB, bool RESIDUAL>
 __global__ void __launch_bounds__(256)
 k_sipg_cartesian(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
@@ -264,6 +328,189 @@ k_sipg_rows(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__
     else
       y[g] = VT(out);
   }
+}
+
+// Line-tiled variant (3D): one thread per x-line (ly, lz) of a cell, its NP
+// values in registers.  The x-direction line operator and the x mass
+// contractions stay in registers; the y and z line operators read the cell's
+// own planes from LDS and the face neighbours' planes (NP contiguous rows)
+// from global memory, shared through the cache by the NP threads of a plane:
+//   y = M_z (M_y z_0 + M_x z_1) + M_y M_x z_2,  z_d = line operator along d.
+template<int NP>
+constexpr int lines_cells_per_block()
+{
+  return 256 / (NP * NP);
+}
+
+template<int NP, typename VT, typename CT, bool RESIDUAL>
+__global__ void __launch_bounds__(256)
+k_sipg_lines(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
+             const OpTab<CT, NP> tab, const int n, const int64_t ncell)
+{
+  constexpr int NL = NP * NP, NDC = NL * NP, CPB = lines_cells_per_block<NP>();
+  __shared__ CT xs[CPB][NDC];  // own cells
+  __shared__ CT zs[CPB][NDC];  // z_0, later M_y z_0 + M_x z_1
+  __shared__ CT ws[CPB][NDC];  // M_x z_2
+  const int cs = threadIdx.x / NL, ln = threadIdx.x % NL;
+  const int ly = ln % NP, lz = ln / NP;
+  const bool in = cs < CPB;
+  const int csafe = in ? cs : 0;
+  const int64_t cell = int64_t(blockIdx.x) * CPB + csafe;
+  const bool act = in && cell < ncell;
+  const int64_t nn = int64_t(n) * n;
+  const int cx = int(cell % n), cy = int((cell / n) % n), cz = int(cell / nn);
+  const int lofs = NP * ly + NL * lz;  // this line within the cell
+  CT xo[NP];
+  if(act)
+    load_row<NP>(x + cell * NDC + lofs, xo);
+  else
+#pragma unroll
+    for(int i = 0; i < NP; ++i) xo[i] = CT(0);
+  if(in)
+#pragma unroll
+    for(int i = 0; i < NP; ++i) xs[cs][lofs + i] = xo[i];
+
+  // z_0: along x, own line and the x-neighbours' lines
+  CT z0[NP];
+  {
+    const bool hl = cx > 0, hr = cx < n - 1;
+    const CT *D = tab.D[(hl ? 1 : 0) | (hr ? 2 : 0)];
+    CT xl[NP], xr[NP];
+#pragma unroll
+    for(int i = 0; i < NP; ++i) xl[i] = xr[i] = CT(0);
+    if(act && hl) load_row<NP>(x + (cell - 1) * NDC + lofs, xl);
+    if(act && hr) load_row<NP>(x + (cell + 1) * NDC + lofs, xr);
+#pragma unroll
+    for(int i = 0; i < NP; ++i)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int j = 0; j < NP; ++j)
+        s += D[i * NP + j] * xo[j] + tab.Fpm[i * NP + j] * xl[j] + tab.Fmp[i * NP + j] * xr[j];
+      z0[i] = s;
+    }
+  }
+  __syncthreads();  // own cells staged
+
+  // z_1 (along y) and z_2 (along z) for every x-node of this line
+  CT z1[NP], z2[NP];
+  {
+    const bool hl = cy > 0, hr = cy < n - 1;
+    const CT *D = tab.D[(hl ? 1 : 0) | (hr ? 2 : 0)];
+#pragma unroll
+    for(int i = 0; i < NP; ++i)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int j = 0; j < NP; ++j) s += D[ly * NP + j] * xs[csafe][i + NP * j + NL * lz];
+      z1[i] = s;
+    }
+#pragma unroll
+    for(int side = 0; side < 2; ++side)
+    {
+      if(!(act && (side == 0 ? hl : hr))) continue;
+      const CT *F = side == 0 ? tab.Fpm : tab.Fmp;
+      const VT *pn = x + (side == 0 ? cell - n : cell + n) * NDC + NL * lz;
+#pragma unroll
+      for(int j = 0; j < NP; ++j)
+      {
+        CT row[NP];
+        load_row<NP>(pn + NP * j, row);
+        const CT f = F[ly * NP + j];
+#pragma unroll
+        for(int i = 0; i < NP; ++i) z1[i] += f * row[i];
+      }
+    }
+  }
+  {
+    const bool hl = cz > 0, hr = cz < n - 1;
+    const CT *D = tab.D[(hl ? 1 : 0) | (hr ? 2 : 0)];
+#pragma unroll
+    for(int i = 0; i < NP; ++i)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int k = 0; k < NP; ++k) s += D[lz * NP + k] * xs[csafe][i + NP * ly + NL * k];
+      z2[i] = s;
+    }
+#pragma unroll
+    for(int side = 0; side < 2; ++side)
+    {
+      if(!(act && (side == 0 ? hl : hr))) continue;
+      const CT *F = side == 0 ? tab.Fpm : tab.Fmp;
+      const VT *pn = x + (side == 0 ? cell - nn : cell + nn) * NDC + NP * ly;
+#pragma unroll
+      for(int k = 0; k < NP; ++k)
+      {
+        CT row[NP];
+        load_row<NP>(pn + NL * k, row);
+        const CT f = F[lz * NP + k];
+#pragma unroll
+        for(int i = 0; i < NP; ++i) z2[i] += f * row[i];
+      }
+    }
+  }
+  // x mass contractions in registers
+  CT m1[NP], w2[NP];
+#pragma unroll
+  for(int i = 0; i < NP; ++i)
+  {
+    CT s1 = 0, s2 = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+    {
+      s1 += tab.M[i * NP + j] * z1[j];
+      s2 += tab.M[i * NP + j] * z2[j];
+    }
+    m1[i] = s1;
+    w2[i] = s2;
+  }
+  if(in)
+#pragma unroll
+    for(int i = 0; i < NP; ++i)
+    {
+      zs[cs][lofs + i] = z0[i];
+      ws[cs][lofs + i] = w2[i];
+    }
+  __syncthreads();
+  CT a[NP], bb[NP];
+#pragma unroll
+  for(int i = 0; i < NP; ++i)
+  {
+    CT s = m1[i], s2 = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+    {
+      const CT m = tab.M[ly * NP + j];
+      s += m * zs[csafe][i + NP * j + NL * lz];
+      s2 += m * ws[csafe][i + NP * j + NL * lz];
+    }
+    a[i] = s;
+    bb[i] = s2;
+  }
+  __syncthreads();
+  if(in)
+#pragma unroll
+    for(int i = 0; i < NP; ++i) zs[cs][lofs + i] = a[i];
+  __syncthreads();
+  CT out[NP];
+#pragma unroll
+  for(int i = 0; i < NP; ++i)
+  {
+    CT s = bb[i];
+#pragma unroll
+    for(int k = 0; k < NP; ++k) s += tab.M[lz * NP + k] * zs[csafe][i + NP * ly + NL * k];
+    out[i] = s;
+  }
+  if(!act) return;
+  if(RESIDUAL)
+  {
+    CT bv[NP];
+    load_row<NP>(b + cell * NDC + lofs, bv);
+#pragma unroll
+    for(int i = 0; i < NP; ++i) out[i] = bv[i] - out[i];
+  }
+  store_row<NP>(y + cell * NDC + lofs, out);
 }
 
 // ----------------------------------------------------------------------------

@@ -24,70 +24,6 @@ namespace tpss
 typedef short tpss_bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
 typedef float tpss_floatx4 __attribute__((__vector_size__(4 * sizeof(float))));
 
-// Contiguous rows of a cell (NP values along x) moved with 16-byte vector
-// accesses; the types only claim element alignment, so any element offset is
-// valid (global memory accesses need no 16-byte alignment on CDNA).
-typedef float tpss_f4u __attribute__((ext_vector_type(4), aligned(4)));
-typedef double tpss_d2u __attribute__((ext_vector_type(2), aligned(8)));
-
-template<int NP, typename VT, typename CT>
-__device__ __forceinline__ void load_row(const VT *__restrict__ p, CT *out)
-{
-  if constexpr(std::is_same<VT, float>::value && NP % 4 == 0)
-  {
-#pragma unroll
-    for(int i = 0; i < NP; i += 4)
-    {
-      const tpss_f4u v = *reinterpret_cast<const tpss_f4u *>(p + i);
-      out[i] = CT(v.x), out[i + 1] = CT(v.y), out[i + 2] = CT(v.z), out[i + 3] = CT(v.w);
-    }
-  }
-  else if constexpr(std::is_same<VT, double>::value && NP % 2 == 0)
-  {
-#pragma unroll
-    for(int i = 0; i < NP; i += 2)
-    {
-      const tpss_d2u v = *reinterpret_cast<const tpss_d2u *>(p + i);
-      out[i] = CT(v.x), out[i + 1] = CT(v.y);
-    }
-  }
-  else
-  {
-#pragma unroll
-    for(int i = 0; i < NP; ++i) out[i] = CT(p[i]);
-  }
-}
-
-template<int NP, typename VT, typename CT>
-__device__ __forceinline__ void store_row(VT *__restrict__ p, const CT *in)
-{
-  if constexpr(std::is_same<VT, float>::value && NP % 4 == 0)
-  {
-#pragma unroll
-    for(int i = 0; i < NP; i += 4)
-    {
-      tpss_f4u v;
-      v.x = float(in[i]), v.y = float(in[i + 1]), v.z = float(in[i + 2]), v.w = float(in[i + 3]);
-      *reinterpret_cast<tpss_f4u *>(p + i) = v;
-    }
-  }
-  else if constexpr(std::is_same<VT, double>::value && NP % 2 == 0)
-  {
-#pragma unroll
-    for(int i = 0; i < NP; i += 2)
-    {
-      tpss_d2u v;
-      v.x = double(in[i]), v.y = double(in[i + 1]);
-      *reinterpret_cast<tpss_d2u *>(p + i) = v;
-    }
-  }
-  else
-  {
-#pragma unroll
-    for(int i = 0; i < NP; ++i) p[i] = VT(in[i]);
-  }
-}
-
 // FP32 -> BF16 bits, round to nearest even (finite inputs)
 __device__ __forceinline__ short bf16_bits(float f)
 {
