@@ -376,17 +376,19 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
       const int jw = min(j0 + pw, patch_end - 1);
       const uint32_t f0 = fac[size_t(jw) * DIM + 0], f1 = fac[size_t(jw) * DIM + 1],
                      f2 = fac[size_t(jw) * DIM + 2];
-      const CT *S0 = S_SCALAR ? S_tab + size_t(f0) * K * K : Ss[plc][0];
-      const CT *S1 = S_SCALAR ? S_tab + size_t(f1) * K * K : Ss[plc][1];
-      const CT *S2 = S_SCALAR ? S_tab + size_t(f2) * K * K : Ss[plc][2];
+      const CT *G0 = S_tab + size_t(f0) * K * K, *G1 = S_tab + size_t(f1) * K * K,
+               *G2 = S_tab + size_t(f2) * K * K;
       // even-odd products where all three 1D problems are mirror-symmetric
-      // (interior patches of a uniform mesh); a wave-uniform branch
+      // (interior patches of a uniform mesh); a wave-uniform branch.  They read
+      // only the first K/2 rows of each matrix, which fit the scalar registers
+      // in FP64 too.
       if(eo_tab && (eo_tab[f0] & eo_tab[f1] & eo_tab[f2]) != 0u)
-        fd_lines<DIM, K, CT, FSY, FSZ, true>(r, o, PAD ? A2[plc] : buf[plc], S0, S1, S2,
+        fd_lines<DIM, K, CT, FSY, FSZ, true>(r, o, PAD ? A2[plc] : buf[plc], G0, G1, G2,
                                              ls[plc][0], ls[plc][1], ls[plc][2], w, t, owner);
       else
-        fd_lines<DIM, K, CT, FSY, FSZ, false>(r, o, PAD ? A2[plc] : buf[plc], S0, S1, S2,
-                                              ls[plc][0], ls[plc][1], ls[plc][2], w, t, owner);
+        fd_lines<DIM, K, CT, FSY, FSZ, false>(
+          r, o, PAD ? A2[plc] : buf[plc], S_SCALAR ? G0 : Ss[plc][0], S_SCALAR ? G1 : Ss[plc][1],
+          S_SCALAR ? G2 : Ss[plc][2], ls[plc][0], ls[plc][1], ls[plc][2], w, t, owner);
     }
     else
       fd_lines<DIM, K, CT, FSY, FSZ>(r, o, PAD ? A2[plc] : buf[plc], Ss[plc][0], Ss[plc][1],
