@@ -106,8 +106,20 @@ constexpr int mvp_min_waves()
   return FD == 1 || !std::is_same<CT, float>::value ? 1 : K <= 8 ? 5 : 3;
 }
 
+// Launch shape of the fused pass: one patch per block where a patch fills a
+// wave (3D, K = 8, vector solves), so that the block is a single wave, its
+// barriers cost nothing and no patch waits for another; otherwise the shape of
+// the other patch kernels (the BF16-MFMA solve pairs two patches per block).
+template<int DIM, int K, int FD>
+struct MvpShape
+{
+  using PS = PatchShape<DIM, K>;
+  static constexpr int PPB = (FD == 0 && PS::L == 64) ? 1 : PS::PPB;
+  static constexpr int THREADS = ((PPB * PS::L + 63) / 64) * 64;
+};
+
 template<int DIM, int K, typename VT, typename CT, int FD = 0>
-__global__ void __launch_bounds__((PatchShape<DIM, K>::THREADS), (mvp_min_waves<K, CT, FD>()))
+__global__ void __launch_bounds__((MvpShape<DIM, K, FD>::THREADS), (mvp_min_waves<K, CT, FD>()))
 k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, K / 2> tab,
                   const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
                   const CT *__restrict__ S_tab, const CT *__restrict__ lam_tab,
@@ -116,7 +128,8 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
                   const int patch_end, const int n, const int x_is_zero)
 {
   using PS = PatchShape<DIM, K>;
-  constexpr int NP = PS::NP, L = PS::L, NPD = PS::NPD, NDC = PS::NDC, PPB = PS::PPB;
+  using MS = MvpShape<DIM, K, FD>;
+  constexpr int NP = PS::NP, L = PS::L, NPD = PS::NPD, NDC = PS::NDC, PPB = MS::PPB;
   // LDS strides (y, z) of the patch buffers, padded for 3D K = 8 so that the
   // line transposes of the solve (FSY, FSZ) and the cross-line reads of the
   // coupling stage (CSY, CSZ) are free of bank conflicts.  The padded solve
@@ -144,7 +157,7 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   // scalar registers (operands of the contraction FMAs) instead of being
   // staged through LDS, which takes most LDS reads off the pass.  (FP64
   // matrices would need twice the scalar registers and spill.)
-  constexpr bool WAVE_PATCH = FD == 0 && L == 64 && PS::THREADS == PPB * 64;
+  constexpr bool WAVE_PATCH = FD == 0 && L == 64 && MS::THREADS == PPB * 64;
   constexpr bool S_SCALAR = WAVE_PATCH && std::is_same<CT, float>::value;
   if constexpr(!S_SCALAR)
     for(int idx = threadIdx.x; idx < PPB * DIM * K * K; idx += blockDim.x)

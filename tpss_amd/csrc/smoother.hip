@@ -465,18 +465,19 @@ void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, b
       tab.Fmp[e] = CT(op.Fmp[e]);
       for(int c = 0; c < 4; ++c) tab.D[c][e] = CT(op.D[c][e]);
     }
-    const unsigned grid = unsigned((end - begin + PS::PPB - 1) / PS::PPB);
+    using MS = MvpShape<DIM, K, FD>;
+    const unsigned grid = unsigned((end - begin + MS::PPB - 1) / MS::PPB);
     if constexpr(FD == 1)
     {
       if constexpr(DIM == 3 && K == 8 && std::is_same<CT, float>::value)
-        hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT, 1>), dim3(grid), dim3(PS::THREADS), 0,
+        hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT, 1>), dim3(grid), dim3(MS::THREADS), 0,
                            s, b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, nullptr, om, relax,
                            begin, end, ls.n, x_is_zero ? 1 : 0);
       else
         throw Error(TPSS_E_UNSUPPORTED, "BF16-MFMA patch solves are built for 3D Q3 (k = 8)");
     }
     else
-      hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT>), dim3(grid), dim3(PS::THREADS), 0, s,
+      hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT>), dim3(grid), dim3(MS::THREADS), 0, s,
                          b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, ls.d_eo.ptr, om, relax,
                          begin, end, ls.n, x_is_zero ? 1 : 0);
     TPSS_HIP(hipGetLastError());
