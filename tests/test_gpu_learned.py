@@ -88,18 +88,22 @@ def test_training_reduces_loss_and_sets_damping():
     assert np.abs(out - expect).max() < 1e-12 * np.abs(expect).max()
 
 
-def test_shipped_model_beats_fixed_damping():
-    """The trained network (tpss_amd/models/damping_dgq3_3d.npy) against the
-    reference's fixed multiplicative damping 1 (TPSS.h:120-163): GMG-CG on a
-    distorted 16^3 DG Q3 mesh (interior vertices moved by 0.2 h) contracts
-    faster and needs fewer iterations to 1e-8."""
+@pytest.mark.parametrize("calibrated,n", [(False, 16), (True, 32)])
+def test_shipped_model_beats_fixed_damping(calibrated, n):
+    """The trained network (tpss_amd/models/damping_dgq3_3d.npy), with its own
+    prediction (16^3 cells) and with the output scale calibrated on a 64^3
+    training mesh (32^3 cells), against the reference's fixed multiplicative
+    damping 1 (TPSS.h:120-163): GMG-CG on a distorted DG Q3 mesh (interior
+    vertices moved by 0.2 h, seed 3) contracts faster and needs fewer iterations
+    to 1e-8."""
     t = _tpss()
     from tpss_amd import learned_damping as ld
-    mlp = ld.load()
-    V = ld.distorted_vertices(3, 16, 0.2, seed=3)
+    mlp = ld.load(scale=ld.OUTPUT_SCALE if calibrated else 1.0)
+    V = ld.distorted_vertices(3, n, 0.2, seed=3)
+    levels = int(np.log2(n // 2)) + 1
     res = {}
     for use in (False, True):
-        g = t.GMGSolver(3, 3, 2, 4, ip_factor=4.0, vertices=V, variant=t.MULTIPLICATIVE,
+        g = t.GMGSolver(3, 3, 2, levels, ip_factor=4.0, vertices=V, variant=t.MULTIPLICATIVE,
                         relaxation=1.0)
         if use:
             ld.apply_to_gmg(g, mlp)

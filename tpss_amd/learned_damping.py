@@ -8,6 +8,7 @@ smoother computes for it with its own local solvers and operator
 device by the library's own forward/backward/Adam kernels.
 
     python -m tpss_amd.learned_damping --out tpss_amd/models/damping_dgq3_3d.npy
+    python -m tpss_amd.learned_damping --out tpss_amd/models/damping_dgq3_3d.npy --calibrate
 """
 import argparse
 import os
@@ -15,7 +16,8 @@ import time
 
 import numpy as np
 
-from . import (F64, MULTIPLICATIVE, COLOR_REDBLACK, DampingMLP, Level, SchwarzSmoother)
+from . import (F32, F64, MULTIPLICATIVE, COLOR_REDBLACK, DampingMLP, DeviceVector, GMGSolver,
+               Level, SchwarzSmoother, fill_random)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "damping_dgq3_3d.npy")
@@ -88,8 +90,43 @@ def train(X, y, epochs=40, batch=2048, lr=2e-3, seed=4, hidden=64):
     return mlp, loss0, loss
 
 
-def load(path=DEFAULT_MODEL, n_in=24, hidden=64):
-    return DampingMLP(n_in, hidden, 0.2, 1.2).load(path)
+# The network predicts the locally optimal damping of each patch (mean 0.90 on
+# 0.2 h meshes).  As a multigrid smoother the V-cycle contracts faster at a
+# lower level of damping, so the output range is scaled by one factor
+# calibrated on a training mesh (calibrate_scale: 64^3 Q3, 0.2 h, seed 201,
+# mixed-precision GMG-CG; the factor with the smallest CG reduction rate over
+# 0.60, 0.64, ..., 0.88).  scale=1 gives the network's own prediction.
+OUTPUT_SCALE = 0.72
+
+
+def load(path=DEFAULT_MODEL, n_in=24, hidden=64, scale=OUTPUT_SCALE):
+    """The damping network with its output range [0.2, 1.2] scaled by scale."""
+    return DampingMLP(n_in, hidden, 0.2 * scale, 1.2 * scale).load(path)
+
+
+def calibrate_scale(mlp, n=64, degree=3, amplitude=0.2, seed=201,
+                    scales=tuple(0.60 + 0.04 * i for i in range(8)), rel_tol=1e-8):
+    """Output scale of an (unscaled) damping network with the smallest GMG-CG
+    reduction rate on one distorted training mesh (mixed precision, MVP with
+    red-black colors): returns (best scale, {scale: (iterations, rate)})."""
+    V = distorted_vertices(3, n, amplitude, seed)
+    levels = int(np.log2(n // 2)) + 1
+    res = {}
+    for s in scales:
+        g = GMGSolver(3, degree, 2, levels, ip_factor=4.0, vertices=V, mg_precision=F32,
+                      variant=MULTIPLICATIVE, coloring=COLOR_REDBLACK, relaxation=1.0)
+        for level in range(1, levels):
+            for post in (False, True):
+                sm = g.smoother(level, post)
+                sm.set_patch_damping(mlp.forward(sm.patch_features()).astype(np.float64) * s)
+        b = DeviceVector(g.n_dofs)
+        fill_random(b, 2)
+        x = DeviceVector.zeros(g.n_dofs)
+        info, _ = g.solve(x, b, rel_tol=rel_tol, max_iter=100, raise_on_fail=False)
+        res[round(float(s), 4)] = (info.iterations, info.reduction_rate)
+        del g
+    best = min(res, key=lambda k: res[k][1])
+    return best, res
 
 
 def apply_to_gmg(gmg, mlp):
@@ -108,7 +145,15 @@ def main():
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--samples", type=int, default=4,
                     help="random errors per patch in the optimal-damping target")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only calibrate the output scale of the model at --out")
     args = ap.parse_args()
+    if args.calibrate:
+        best, res = calibrate_scale(load(args.out, scale=1.0))
+        for sc, (it, rate) in res.items():
+            print(f"scale {sc:.2f}: {it} CG iterations, reduction rate {rate:.4f}", flush=True)
+        print(f"calibrated output scale: {best:.2f}", flush=True)
+        return
     t0 = time.time()
     amps = [float(a) for a in args.amplitudes.split(",")]
     X, y = generate(3, args.degree, args.n, amps, n_samples=args.samples)
