@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-gmg", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the distorted-mesh GMG-CG with learned vs fixed damping")
+    ap.add_argument("--no-fit", action="store_true",
+                    help="skip the 48^3 Q4 distorted GMG-CG with surrogate vs fitted local solvers")
     ap.add_argument("--no-elasticity", action="store_true",
                     help="skip the 3D DG Q2 linear elasticity smoother and GMG-CG")
     ap.add_argument("--variants", default="f64,f32,bf16",
@@ -141,6 +143,36 @@ def elasticity_section(t, args, n=64, p=2, mu=1.0, lam=10.0, ip=10.0):
             out[name] = {"seconds": info.seconds, "iterations": info.iterations,
                          "n_frac": info.n_frac, "reduction_rate": info.reduction_rate,
                          "final_rel_residual": hist[-1] / hist[0]}
+            del g
+    return out
+
+
+def fit_section(t, n=48, p=4, n_root=3, amplitude=0.15):
+    """3D SIPG DG Q4 on 48^3 cells whose 3^3 root mesh is distorted by 0.15 h_root
+    and refined (the reference's construction, so the GMG levels are nested),
+    ip x4: mixed-precision GMG-CG to 1e-8 with surrogate-length and with fitted
+    (separable Kronecker fit of the exact local matrices) patch solvers, for the
+    multiplicative smoother and the additive one with the reference's 3D damping."""
+    from tpss_amd import meshes
+    V = meshes.nested_distorted_vertices(3, n_root, n, amplitude, seed=3)
+    levels = int(round(np.log2(n // n_root))) + 1
+    out = {"workload": f"3D SIPG DG Q{p} {n}^3 cells, {n_root}^3 root mesh distorted by "
+                       f"{amplitude} h_root and refined, mixed-precision GMG-CG to 1e-8"}
+    for ls_name, ls in (("surrogate", t.LOCAL_SURROGATE), ("fit", t.LOCAL_FIT)):
+        for sm_name, var, relax in (("mvp", t.MULTIPLICATIVE, 1.0), ("avp_0.15", t.ADDITIVE, 0.15)):
+            t.synchronize()
+            t0 = time.perf_counter()
+            g = t.GMGSolver(3, p, n_root, levels, ip_factor=4.0, vertices=V, mg_precision=t.F32,
+                            variant=var, relaxation=relax, local_solver=ls)
+            t.synchronize()
+            setup = time.perf_counter() - t0
+            b = t.DeviceVector(g.n_dofs)
+            t.fill_random(b, 2)
+            x = t.DeviceVector.zeros(g.n_dofs)
+            info, hist = g.solve(x, b, rel_tol=1e-8, max_iter=200, raise_on_fail=False)
+            out[f"{ls_name}_{sm_name}"] = {"setup_seconds": setup, "seconds": info.seconds,
+                                           "iterations": info.iterations,
+                                           "reduction_rate": info.reduction_rate}
             del g
     return out
 
@@ -257,6 +289,10 @@ def main():
                              "final_rel_residual": hist[-1] / hist[0]}
             del g
 
+    fitted = {}
+    if not args.no_gmg and not args.no_fit:
+        fitted = fit_section(t)
+
     elast = {}
     if not args.no_elasticity:
         elast = elasticity_section(t, args)
@@ -298,6 +334,7 @@ def main():
                             "kernels": r["kernels"]} for name, r in results.items()},
         "gmg_cg": gmg,
         "gmg_cg_distorted_learned_damping": learned,
+        "gmg_cg_distorted_fitted_solvers": fitted,
         "elasticity": elast,
     }
     print(json.dumps(out))

@@ -69,3 +69,47 @@ def test_fit_gmg_converges():
             err = g.smoother(3).fit_errors()
             res["fit_err"] = (float(err.max()), int((err < 0).sum()))
     assert res["fit"][0] > 0 and res["fit"][2] <= 1e-8, res
+
+
+def test_nested_distorted_gmg_matches_oracle():
+    """The reference's distorted meshes (root mesh distorted, then refined;
+    apps/mesh.h:153-204) give nested GMG levels: one AVP V-cycle and the MVP
+    GMG-CG on 3D DG Q2, root 2^3 cells distorted by 0.15 h_root, 8^3 fine cells."""
+    t = _tpss()
+    from tpss_amd import meshes
+    V = meshes.nested_distorted_vertices(3, 2, 8, 0.15, seed=3)
+    b = o.splitmix_uniform(2, 8 ** 3 * 27)
+    g = t.GMGSolver(3, 2, 2, 3, ip_factor=4.0, vertices=V, variant=t.ADDITIVE,
+                    relaxation=0.15)
+    ref = o.GMG(3, 2, 2, 3, ip=4.0, variant=t.ADDITIVE, coloring=t.COLOR_NONE,
+                relaxation=0.15, vertices=V)
+    z = dev(np.zeros(g.n_dofs))
+    g.vcycle(z, dev(b))
+    assert rel_err(host(z), ref.vcycle(b)) < 1e-11
+    g = t.GMGSolver(3, 2, 2, 3, ip_factor=4.0, vertices=V)
+    ref = o.GMG(3, 2, 2, 3, ip=4.0, vertices=V)
+    x = dev(np.zeros(g.n_dofs))
+    info, hist = g.solve(x, dev(b), rel_tol=1e-8)
+    xr, itr, histr = ref.cg(b, rel_tol=1e-8)
+    assert info.iterations == itr
+    assert rel_err(host(x), xr) < 1e-9
+
+
+def test_fit_improves_additive_smoother_on_nested_meshes():
+    """On a nested distorted hierarchy (3D DG Q3, root 3^3 cells distorted by
+    0.15 h_root, 12^3 fine cells) the additive vertex-patch smoother with the
+    reference's 3D damping 0.15 converges, and the fitted local solvers need
+    fewer GMG-CG iterations than the surrogate-length ones."""
+    t = _tpss()
+    from tpss_amd import meshes
+    V = meshes.nested_distorted_vertices(3, 3, 12, 0.15, seed=3)
+    res = {}
+    for name, ls in (("surrogate", t.LOCAL_SURROGATE), ("fit", t.LOCAL_FIT)):
+        g = t.GMGSolver(3, 3, 3, 3, ip_factor=4.0, vertices=V, variant=t.ADDITIVE,
+                        relaxation=0.15, local_solver=ls)
+        b = o.splitmix_uniform(2, g.n_dofs)
+        x = dev(np.zeros(g.n_dofs))
+        info, hist = g.solve(x, dev(b), rel_tol=1e-8, max_iter=200, raise_on_fail=False)
+        res[name] = (info.iterations, info.reduction_rate)
+    assert res["surrogate"][0] > 0 and res["fit"][0] > 0, res
+    assert res["fit"][0] < res["surrogate"][0], res

@@ -18,37 +18,10 @@ import numpy as np
 
 from . import (F32, F64, MULTIPLICATIVE, COLOR_REDBLACK, DampingMLP, DeviceVector, GMGSolver,
                Level, SchwarzSmoother, fill_random)
+from .meshes import distorted_vertices  # noqa: F401 -- the training meshes
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "damping_dgq3_3d.npy")
-
-
-def distorted_vertices(dim, n, amplitude, seed, graded=False):
-    """Interior vertices moved by (amplitude/n) * U[-1,1]^dim; U from SplitMix64
-    keyed by (seed, vertex*dim + d); boundary vertices fixed.  graded=True scales
-    the amplitude by the vertex's x coordinate (no distortion at x = 0, the full
-    amplitude at x = 1)."""
-    g = np.linspace(0.0, 1.0, n + 1)
-    mesh = np.meshgrid(*([g] * dim), indexing="ij")
-    v = np.stack([m.transpose(tuple(range(dim))[::-1]).ravel() for m in mesh], axis=-1)
-    nv = v.shape[0]
-    i = np.arange(nv * dim, dtype=np.uint64)
-    with np.errstate(over="ignore"):
-        z = np.uint64(seed) * np.uint64(0x100000000) + i + np.uint64(0x9E3779B97F4A7C15)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    noise = (2.0 * ((z >> np.uint64(11)).astype(np.float64) / 9007199254740992.0) - 1.0).reshape(nv, dim)
-    idx = np.arange(nv)
-    interior = np.ones(nv, dtype=bool)
-    t = idx.copy()
-    for _ in range(dim):
-        c = t % (n + 1)
-        interior &= (c > 0) & (c < n)
-        t //= n + 1
-    amp = amplitude * (v[:, :1] if graded else 1.0)
-    v[interior] += ((amp / n) * noise)[interior]
-    return v
 
 
 def patch_samples(dim, degree, n, amplitude, seed, ip_factor=4.0, n_samples=4,
