@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 # HBM bytes per launch of the FP32 fused color pass on the 64^3 Q3 level, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a copy kernel of
 # known size): profiles/mvp_fused_pass_64cubed_q3_hbm_traffic.md
-FUSED_PASS_TRAFFIC_BYTES = {(64, 3): 2.0 * 47811.0 * 1024 + 39053.9 * 1024}
+FUSED_PASS_TRAFFIC_BYTES = {(64, 3): 2.0 * 52113.6 * 1024 + 31255.9 * 1024}
 
 
 def parse():
