@@ -94,10 +94,15 @@ double read_scalar(const double *dptr, hipStream_t s)
 // stop when ||r_k|| <= tol ||r_0||).  Scalars stay on the device (sc: 5
 // values of this solve's own, the coarse CG runs inside the outer CG's
 // preconditioner); one host synchronisation per iteration for the
-// convergence test.
+// convergence test.  precond(z, r, rz, r_low_ready) sets z = B r and the
+// device scalar rz = r.z.  The update x += alpha p, r -= alpha Ap forms r.r in
+// the same pass and, if r_low is given, writes the updated r in FP32 there
+// (the preconditioner is then told that its FP32 input is ready).
+using Precond = std::function<void(double *, const double *, double *, bool)>;
+
 int pcg(tpss_gmg *g, tpss_level *A, double *x, const double *b, double tol, int maxit,
-        const std::function<void(double *, const double *)> &precond, hipStream_t s,
-        std::vector<double> *hist, double *r, double *z, double *p, double *Ap, double *sc)
+        const Precond &precond, hipStream_t s, std::vector<double> *hist, double *r, double *z,
+        double *p, double *Ap, double *sc, float *r_low)
 {
   const int64_t N = A->N;
   // sc: [rz, pAp, rr, rz_new, rz_old]
@@ -107,30 +112,37 @@ int pcg(tpss_gmg *g, tpss_level *A, double *x, const double *b, double tol, int 
   const double r0 = std::sqrt(read_scalar(sc + 2, s));
   if(hist) hist->assign(1, r0);
   if(r0 == 0.0) return 0;
-  if(precond)
-    precond(z, r);
-  else
-    TPSS_HIP(hipMemcpyAsync(z, r, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  auto apply_precond = [&](double *rz, bool r_low_ready) {
+    if(precond)
+      precond(z, r, rz, r_low_ready);
+    else
+    {
+      TPSS_HIP(hipMemcpyAsync(z, r, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      dot64(g, r, z, N, rz, s);
+    }
+  };
+  apply_precond(sc + 0, false);
   TPSS_HIP(hipMemcpyAsync(p, z, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-  dot64(g, r, z, N, sc + 0, s);
   const unsigned grid = blocks_for(N, 256);
+  const int nb = DotWorkspace::blocks(N);
   for(int it = 1; it <= maxit; ++it)
   {
     operator_apply_any(A, Ap, p, nullptr, nullptr, 0, s);
     dot64(g, p, Ap, N, sc + 1, s);
-    hipLaunchKernelGGL(k_cg_update, dim3(grid), dim3(256), 0, s, x, r, p, Ap, N, sc);
+    if(r_low)
+      hipLaunchKernelGGL(k_cg_update_dot<true>, dim3(nb), dim3(256), 0, s, x, r, p, Ap, N, sc,
+                         g->dots.partial.ptr, r_low);
+    else
+      hipLaunchKernelGGL(k_cg_update_dot<false>, dim3(nb), dim3(256), 0, s, x, r, p, Ap, N, sc,
+                         g->dots.partial.ptr, nullptr);
     TPSS_HIP(hipGetLastError());
-    dot64(g, r, r, N, sc + 2, s);
+    g->dots.finish(nb, sc + 2, s);
     const double rn = std::sqrt(read_scalar(sc + 2, s));
     if(hist) hist->push_back(rn);
     if(!std::isfinite(rn)) throw Error(TPSS_E_NONFINITE, "non-finite residual in CG");
     if(rn <= tol * r0) return it;
-    if(precond)
-      precond(z, r);
-    else
-      TPSS_HIP(hipMemcpyAsync(z, r, N * sizeof(double), hipMemcpyDeviceToDevice, s));
     TPSS_HIP(hipMemcpyAsync(sc + 4, sc + 0, sizeof(double), hipMemcpyDeviceToDevice, s));
-    dot64(g, r, z, N, sc + 3, s);
+    apply_precond(sc + 3, r_low != nullptr);
     hipLaunchKernelGGL(k_cg_direction, dim3(grid), dim3(256), 0, s, p, z, N, sc + 3);
     TPSS_HIP(hipGetLastError());
     TPSS_HIP(hipMemcpyAsync(sc + 0, sc + 3, sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -222,14 +234,17 @@ void coarse_solve(tpss_gmg *g, double *x, const double *b, hipStream_t s)
     return;
   }
   const int it = pcg(g, g->coarse64, x, b, g->desc.coarse_tol, 100000, nullptr, s, nullptr,
-                     g->cr.ptr, g->cz.ptr, g->cp.ptr, g->cAp.ptr, g->scal.ptr + 8);
+                     g->cr.ptr, g->cz.ptr, g->cp.ptr, g->cAp.ptr, g->scal.ptr + 8, nullptr);
   if(it < 0) throw Error(TPSS_E_NOT_CONVERGED, "coarse CG did not converge");
 }
 
-void vcycle(tpss_gmg *g, int l, hipStream_t s)
+// one V-cycle x_l = B_l b_l; xo / bo (optional) stand in for the level's own
+// x_l / b_l (an FP64 hierarchy runs its finest level on the CG's vectors)
+void vcycle(tpss_gmg *g, int l, hipStream_t s, void *xo = nullptr, const void *bo = nullptr)
 {
   tpss_level *lv = g->levels[l];
-  void *x = g->vx[l].ptr, *b = g->vb[l].ptr, *r = g->vr[l].ptr;
+  void *x = xo ? xo : g->vx[l].ptr, *r = g->vr[l].ptr;
+  const void *b = bo ? bo : g->vb[l].ptr;
   if(l == 0)
   {
     convert(g->cb64.ptr, TPSS_F64, b, g->mg_prec, lv->N, s);
@@ -250,14 +265,36 @@ void vcycle(tpss_gmg *g, int l, hipStream_t s)
   smoother_step_any(g->post[l], x, b, s);
 }
 
-// z = B r on the fine level; r, z FP64
-void apply_mg(tpss_gmg *g, double *z, const double *r, hipStream_t s)
+// z = B r on the fine level; r, z FP64.  rz (optional device scalar)
+// receives r.z, formed in the pass that brings z to FP64; r_low_ready: the
+// finest level's right-hand side already holds r in the multigrid precision.
+void apply_mg(tpss_gmg *g, double *z, const double *r, hipStream_t s, double *rz = nullptr,
+              bool r_low_ready = false)
 {
   const int l = g->L - 1;
   const int64_t N = g->levels[l]->N;
-  convert(g->vb[l].ptr, g->mg_prec, r, TPSS_F64, N, s);
+  if(g->mg_prec == TPSS_F64 && z != r)
+  {
+    vcycle(g, l, s, z, r);
+    if(rz) dot64(g, r, z, N, rz, s);
+    return;
+  }
+  if(!r_low_ready) convert(g->vb[l].ptr, g->mg_prec, r, TPSS_F64, N, s);
   vcycle(g, l, s);
-  convert(z, TPSS_F64, g->vx[l].ptr, g->mg_prec, N, s);
+  if(!rz)
+  {
+    convert(z, TPSS_F64, g->vx[l].ptr, g->mg_prec, N, s);
+    return;
+  }
+  const int nb = DotWorkspace::blocks(N);
+  if(g->mg_prec == TPSS_F32)
+    hipLaunchKernelGGL(k_convert_dot<float>, dim3(nb), dim3(256), 0, s, z,
+                       (const float *)g->vx[l].ptr, r, N, g->dots.partial.ptr);
+  else
+    hipLaunchKernelGGL(k_convert_dot<double>, dim3(nb), dim3(256), 0, s, z,
+                       (const double *)g->vx[l].ptr, r, N, g->dots.partial.ptr);
+  TPSS_HIP(hipGetLastError());
+  g->dots.finish(nb, rz, s);
 }
 }  // namespace
 }  // namespace tpss
@@ -397,9 +434,12 @@ int tpss_gmg_solve(tpss_gmg *g, void *x, const void *b, double rel_tol, int max_
     TPSS_HIP(hipStreamSynchronize(s));
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<double> hist;
-    auto pre = [&](double *z, const double *r) { apply_mg(g, z, r, s); };
+    auto pre = [&](double *z, const double *r, double *rz, bool r_low_ready) {
+      apply_mg(g, z, r, s, rz, r_low_ready);
+    };
+    float *r_low = g->mg_prec == TPSS_F32 ? (float *)g->vb[g->L - 1].ptr : nullptr;
     const int it = pcg(g, g->fine64, (double *)x, (const double *)b, rel_tol, max_iter, pre, s,
-                       &hist, g->r.ptr, g->z.ptr, g->p.ptr, g->Ap.ptr, g->scal.ptr);
+                       &hist, g->r.ptr, g->z.ptr, g->p.ptr, g->Ap.ptr, g->scal.ptr, r_low);
     TPSS_HIP(hipStreamSynchronize(s));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if(residual_history)

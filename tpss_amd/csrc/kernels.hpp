@@ -964,18 +964,63 @@ __global__ void k_nonfinite(const T *__restrict__ x, const int64_t n, int *__res
   if(i < n && !isfinite(double(x[i]))) flag[0] = 1;
 }
 
-// CG helpers with device scalars: x += (s[0]/s[1]) p ; r -= (s[0]/s[1]) Ap
-static __global__ void k_cg_update(double *__restrict__ x, double *__restrict__ r,
-                            const double *__restrict__ p, const double *__restrict__ Ap,
-                            const int64_t n, const double *__restrict__ s)
+// sum of one value per thread over a block of 256 threads (the tree of
+// k_dot_partial); the result is valid in thread 0
+__device__ __forceinline__ double block_sum_256(double v, double *sh)
 {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for(int w = 128; w > 0; w >>= 1)
+  {
+    if(threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// CG update with device scalars, alpha = s[0]/s[1]:  x += alpha p,
+// r -= alpha Ap, block partial sums of the updated r.r (the grid-stride
+// order of k_dot_partial, so the sums match dot(r, r) bit for bit) and, if
+// r_low is given, the updated r in FP32 (the right-hand side of an FP32 V-cycle)
+template<bool LOW>
+__global__ void __launch_bounds__(256)
+k_cg_update_dot(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                const double *__restrict__ Ap, const int64_t n, const double *__restrict__ s,
+                double *__restrict__ partial, float *__restrict__ r_low)
+{
+  __shared__ double sh[256];
   const double alpha = s[0] / s[1];
-  if(i < n)
+  double acc = 0.0;
+  for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+      i += int64_t(gridDim.x) * blockDim.x)
   {
     x[i] += alpha * p[i];
-    r[i] -= alpha * Ap[i];
+    const double ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    if(LOW) r_low[i] = float(ri);
+    acc += ri * ri;
   }
+  const double tot = block_sum_256(acc, sh);
+  if(threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// z = FP64(v) with block partial sums of r.z (grid-stride order of k_dot_partial)
+template<typename T>
+__global__ void __launch_bounds__(256)
+k_convert_dot(double *__restrict__ z, const T *__restrict__ v, const double *__restrict__ r,
+              const int64_t n, double *__restrict__ partial)
+{
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+      i += int64_t(gridDim.x) * blockDim.x)
+  {
+    const double zi = double(v[i]);
+    z[i] = zi;
+    acc += r[i] * zi;
+  }
+  const double tot = block_sum_256(acc, sh);
+  if(threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
 // p = z + (s[0]/s[1]) p

@@ -50,11 +50,18 @@ struct DotWorkspace
   DeviceArray<double> partial;
   static constexpr int NB = 1024;
   DotWorkspace() : partial(NB) {}
+  // blocks of 256 threads of a grid-stride pass over n entries
+  static int blocks(int64_t n) { return int(std::min<int64_t>(NB, std::max<int64_t>(1, (n + 255) / 256))); }
   template<typename T>
   void dot(const T *a, const T *b, int64_t n, double *out, hipStream_t s)
   {
-    const int nb = int(std::min<int64_t>(NB, std::max<int64_t>(1, (n + 255) / 256)));
+    const int nb = blocks(n);
     hipLaunchKernelGGL((k_dot_partial<T>), dim3(nb), dim3(256), 0, s, a, b, n, partial.ptr);
+    finish(nb, out, s);
+  }
+  // out = the sum of the first nb partial sums (written by a pass of nb blocks)
+  void finish(int nb, double *out, hipStream_t s)
+  {
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, partial.ptr, nb, out);
     TPSS_HIP(hipGetLastError());
   }
