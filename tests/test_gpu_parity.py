@@ -142,9 +142,10 @@ def test_fp32_patch_solves(variant):
     assert rel_err(host(d64), ref.precond_vmult(src)) < 2e-5
 
 
-def test_per_patch_damping():
+@pytest.mark.parametrize("dim,p,n,mult", [(2, 3, 12, False), (3, 3, 5, True)])
+def test_per_patch_damping(dim, p, n, mult):
     import tpss_amd as t
-    lv, sm, ref = make(2, 3, 12, t.ADDITIVE)
+    lv, sm, ref = make(dim, p, n, t.MULTIPLICATIVE if mult else t.ADDITIVE)
     omega = 0.2 + 0.5 * (o.splitmix_uniform(9, sm.n_patches) + 1.0) / 2.0
     omega = omega.astype(np.float32).astype(np.float64)  # stored as FP32 on the device
     sm.set_patch_damping(omega)
@@ -298,7 +299,8 @@ def test_distorted_gmg_matches_oracle():
 @pytest.mark.parametrize("dim,p,n,relax,rev", [(2, 3, 9, 1.0, False), (3, 3, 7, 1.0, False),
                                                (3, 2, 6, 0.7, True), (2, 2, 8, 0.6, False),
                                                (2, 3, 4, 1.0, False), (2, 3, 8, 1.0, True),
-                                               (2, 3, 16, 1.0, False), (3, 2, 4, 1.0, False)])
+                                               (2, 3, 16, 1.0, False), (3, 2, 4, 1.0, False),
+                                               (3, 3, 6, 0.8, True)])
 def test_mvp_fused_and_residual_paths(dim, p, n, relax, rev):
     """Residual-free color passes and the residual-update path both match the
     oracle (which recomputes the full residual before every color)."""
@@ -341,10 +343,12 @@ def test_gmg_bf16_reaches_tolerance():
 
 
 @pytest.mark.parametrize("dim,p,n,rev,n_steps", [(2, 3, 9, False, 1), (3, 2, 6, True, 2),
-                                                 (3, 3, 5, False, 1)])
+                                                 (3, 3, 5, False, 1), (3, 3, 24, True, 1)])
 def test_residual_free_richardson_step(dim, p, n, rev, n_steps):
     """With omega_R = 1 the multiplicative step runs as color passes on (x, b)
-    without a residual update; it must match the reference's residual + sweep."""
+    without a residual update; it must match the reference's residual + sweep.
+    3D Q3: colors of up to 512 patches (n = 5) run the four-wave pass, larger
+    ones (n = 24: about 760 patches per color) one wave per patch."""
     import tpss_amd as t
     lv, sm, ref = make(dim, p, n, t.MULTIPLICATIVE, reverse=rev, n_steps=n_steps)
     b = o.splitmix_uniform(4, ref.N)
