@@ -31,6 +31,35 @@ __device__ __forceinline__ void load_pair(const VT *p, CT *o)
   o[1] = CT(p[1]);
 }
 
+// value of lane sel_i of the lane's quad (four consecutive lanes) for a DPP
+// quad_perm control CTRL = sel_0 | sel_1 << 2 | sel_2 << 4 | sel_3 << 6
+template<int CTRL>
+__device__ __forceinline__ float quad_perm(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template<int CTRL>
+__device__ __forceinline__ double quad_perm(double v)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// the 8 entries of a line held as two per lane by the four lanes of a quad
+template<typename CT>
+__device__ __forceinline__ void quad_gather(const CT *mine, CT *all)
+{
+  all[0] = quad_perm<0x00>(mine[0]);
+  all[1] = quad_perm<0x00>(mine[1]);
+  all[2] = quad_perm<0x55>(mine[0]);
+  all[3] = quad_perm<0x55>(mine[1]);
+  all[4] = quad_perm<0xAA>(mine[0]);
+  all[5] = quad_perm<0xAA>(mine[1]);
+  all[6] = quad_perm<0xFF>(mine[0]);
+  all[7] = quad_perm<0xFF>(mine[1]);
+}
+
 template<typename VT, typename CT>
 __global__ void __launch_bounds__(256)
 k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 4> tab,
@@ -44,6 +73,7 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   constexpr int CSY = 12, CSZ = 96;  // coupling buffers
   __shared__ CT A1[K * CSZ];
   __shared__ CT A2[K * CSZ];  // also the solve buffer
+  __shared__ CT A3[K * CSZ];
   __shared__ CT Ss[3][K * K];
   __shared__ CT ls[3][K];
 
@@ -118,7 +148,7 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
           z[1] += f * row[1];
         }
       }
-      const CT zo0 = __shfl_xor(z[0], 1), zo1 = __shfl_xor(z[1], 1);
+      const CT zo0 = quad_perm<0xB1>(z[0]), zo1 = quad_perm<0xB1>(z[1]);  // lane ^ 1
       CT zc[NP];
 #pragma unroll
       for(int jj = 0; jj < NP; ++jj)
@@ -163,9 +193,8 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
       a[k] = s;
       bb[k] = s2;
     }
-    __syncthreads();
 #pragma unroll
-    for(int k = 0; k < 2; ++k) Z[e0 + k + CSY * ty + CSZ * tz] = a[k];
+    for(int k = 0; k < 2; ++k) A3[e0 + k + CSY * ty + CSZ * tz] = a[k];
     __syncthreads();
     const int z00 = bz * NP;
 #pragma unroll
@@ -173,22 +202,19 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
     {
       CT s = bb[k];
 #pragma unroll
-      for(int kk = 0; kk < NP; ++kk) s += tab.M[lz * NP + kk] * Z[e0 + k + CSY * ty + CSZ * (z00 + kk)];
+      for(int kk = 0; kk < NP; ++kk) s += tab.M[lz * NP + kk] * A3[e0 + k + CSY * ty + CSZ * (z00 + kk)];
       g[k] -= s;
     }
   }
 
   // fast-diagonalization solve on the patch in B (the coupling's reads of A2
-  // precede its last two barriers)
+  // precede its last barrier)
   CT *B = A2;
   CT rr[K];
   CT o[2];
-#pragma unroll
-  for(int k = 0; k < 2; ++k) B[e0 + k + SY * ty + SZ * tz] = g[k];
-  __syncthreads();  // also publishes Ss / ls
-  // forward x: the x-line (ty, tz)
-#pragma unroll
-  for(int a = 0; a < K; ++a) rr[a] = B[a + SY * ty + SZ * tz];
+  if(x_is_zero) __syncthreads();  // publishes Ss / ls (else the coupling's barriers do)
+  // forward x: the x-line (ty, tz), gathered from the quad
+  quad_gather(g, rr);
 #pragma unroll
   for(int k = 0; k < 2; ++k)
   {
@@ -229,12 +255,8 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
       o[k] = eig_scale(s, w, lxy + ls[2][e0 + k]);
     }
   }
-#pragma unroll
-  for(int k = 0; k < 2; ++k) B[p0 + SY * p1 + SZ * (e0 + k)] = o[k];
-  __syncthreads();
-  // backward z
-#pragma unroll
-  for(int a = 0; a < K; ++a) rr[a] = B[p0 + SY * p1 + SZ * a];
+  // backward z: the scaled z-line gathered from the quad
+  quad_gather(o, rr);
 #pragma unroll
   for(int k = 0; k < 2; ++k)
   {
