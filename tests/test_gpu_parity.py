@@ -300,7 +300,8 @@ def test_distorted_gmg_matches_oracle():
                                                (3, 2, 6, 0.7, True), (2, 2, 8, 0.6, False),
                                                (2, 3, 4, 1.0, False), (2, 3, 8, 1.0, True),
                                                (2, 3, 16, 1.0, False), (3, 2, 4, 1.0, False),
-                                               (3, 3, 6, 0.8, True)])
+                                               (3, 3, 6, 0.8, True), (3, 3, 2, 1.0, False),
+                                               (3, 3, 3, 1.0, True)])
 def test_mvp_fused_and_residual_paths(dim, p, n, relax, rev):
     """Residual-free color passes and the residual-update path both match the
     oracle (which recomputes the full residual before every color)."""
