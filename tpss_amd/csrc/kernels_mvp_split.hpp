@@ -13,9 +13,12 @@
 // entries e = 2 (t % 4) and e + 1 of that line (x-cell t % 4 / 2, cell-local
 // x 2 (t % 2) and 2 (t % 2) + 1).  In the y and z stages of the solve the
 // same thread takes entries 2 (t % 4), 2 (t % 4) + 1 of the y- or z-line
-// t / 4.  The four threads of a line are consecutive lanes of one wave, so a
-// line is read and rewritten in place by one wave; lines pass between waves
-// through LDS behind block barriers.
+// t / 4.  The four threads of a line are consecutive lanes of one wave (a
+// quad): where a stage needs the whole line it takes it from the quad with DPP
+// quad permutations (the x-line entering the solve, the z-line between the
+// forward and backward z products), otherwise a line is read and rewritten in
+// place by its quad; lines pass between waves through LDS behind block
+// barriers (six per patch with the outer-face coupling).
 #pragma once
 
 #include <hip/hip_runtime.h>
