@@ -17,7 +17,8 @@
 //     integration against the test gradients (z, y, x stages);
 //   faces, one direction at a time (both faces of the cell): traces of both
 //     sides contracted along the normal (the neighbour's straight from global
-//     memory), fluxes per face point, fluxes projected on the tangential test
+//     memory; the face neighbours' cell indices are formed once per block),
+//     fluxes per face point, fluxes projected on the tangential test
 //     functions, and every thread adds the terms of its test function.
 // Faces are evaluated element-centrically (every output entry is written by
 // exactly one thread).
@@ -74,6 +75,7 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
   // projections of the fluxes on the tangential test functions ([.][0][.][.])
   __shared__ T PA[CPB][2][2][DIM][NF], PB[CPB][2][2][DIM][NF];
   __shared__ T FA[CPB][2][DIM][NF], FB[CPB][2][DIM][DIM][NF];
+  __shared__ int64_t nbc[CPB][2 * DIM];  // face neighbours [2 D + side] (-1: boundary face)
   __shared__ ElastTab<NP, T> tl;
 
   const int t = threadIdx.x;
@@ -95,6 +97,19 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
     T *dst = reinterpret_cast<T *>(&tl);
     constexpr int NTAB = int(sizeof(ElastTab<NP, T>) / sizeof(T));
     for(int i = t; i < NTAB; i += NT) dst[i] = src[i];
+  }
+  if(t < CPB * 2 * DIM)
+  {
+    const int s = t / (2 * DIM), f = t % (2 * DIM), D = f >> 1, side = f & 1;
+    const int64_t ce = cell_of(s);
+    int64_t nb = -1;
+    if(ce >= 0)
+    {
+      const int64_t cstride = D == 0 ? int64_t(1) : D == 1 ? int64_t(n) : int64_t(n) * n;
+      const int cd = int((ce / cstride) % n);
+      if(side == 1 ? cd < n - 1 : cd > 0) nb = side == 1 ? ce + cstride : ce - cstride;
+    }
+    nbc[s][f] = nb;
   }
   if(act)
 #pragma unroll
@@ -291,11 +306,10 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
   }
 
   // ---------------- faces, direction D fixed at compile time (both faces)
-  auto face_dir = [&](auto Dc, int64_t cstride) {
+  auto face_dir = [&](auto Dc) {
     constexpr int D = decltype(Dc)::value;
     constexpr int E1 = D == 0 ? 1 : 0;
     constexpr int E2 = DIM == 3 ? (D == 2 ? 1 : 2) : E1;
-    auto coord = [&](int64_t ce) { return int((ce / cstride) % n); };
     __syncthreads();  // PA / PB of the previous direction have been read
     // traces along the normal: value (PA) and normal derivative (PB) of both
     // cell sides for every component and tangential node
@@ -304,10 +318,8 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
       const int s = o / (2 * 2 * DIM * NF), r = o % (2 * 2 * DIM * NF);
       const int side = r / (2 * DIM * NF), sd = (r / (DIM * NF)) % 2;
       const int c = (r / NF) % DIM, m = r % NF;
-      const int64_t ce = cell_of(s);
-      if(ce < 0) continue;
-      const int cd = coord(ce);
-      const bool inner = side == 1 ? cd < n - 1 : cd > 0;
+      if(cell_of(s) < 0) continue;
+      const bool inner = nbc[s][2 * D + side] >= 0;
       if(sd == 1 && !inner) continue;
       const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
       const int base = i1 * ST[E1] + (DIM == 3 ? i2 * ST[E2] : 0);
@@ -325,8 +337,7 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
       }
       else
       {
-        const int64_t nb = side == 1 ? ce + cstride : ce - cstride;
-        const VT *src = x + c * Ns + nb * NDC + base;
+        const VT *src = x + c * Ns + nbc[s][2 * D + side] * NDC + base;
 #pragma unroll
         for(int id = 0; id < NP; ++id)
         {
@@ -343,10 +354,8 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
     for(int o = t; o < CPB * 2 * NF; o += NT)
     {
       const int s = o / (2 * NF), side = (o / NF) % 2, l = o % NF;
-      const int64_t ce = cell_of(s);
-      if(ce < 0) continue;
-      const int cd = coord(ce);
-      const bool inner = side == 1 ? cd < n - 1 : cd > 0;
+      if(cell_of(s) < 0) continue;
+      const bool inner = nbc[s][2 * D + side] >= 0;
       const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
       // traces and physical gradients of this cell (o) and the exterior state (x)
       T uo[DIM], go[DIM][DIM], ux[DIM], gx[DIM][DIM];
@@ -464,10 +473,9 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
       }
     }
   };
-  const int64_t nn = int64_t(n) * n;
-  face_dir(std::integral_constant<int, 0>{}, int64_t(1));
-  face_dir(std::integral_constant<int, 1>{}, int64_t(n));
-  if constexpr(DIM == 3) face_dir(std::integral_constant<int, DIM == 3 ? 2 : 1>{}, nn);
+  face_dir(std::integral_constant<int, 0>{});
+  face_dir(std::integral_constant<int, 1>{});
+  if constexpr(DIM == 3) face_dir(std::integral_constant<int, DIM == 3 ? 2 : 1>{});
 
   if(act)
 #pragma unroll
