@@ -118,12 +118,12 @@ k_sipg_cartesian(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restr
   int lid[DIM], cid[DIM];
   {
     int t = li;
-    int64_t c = cell;
+    int c = int(cell);  // cells < n^3 <= 1000^3 < 2^31
     for(int d = 0; d < DIM; ++d)
     {
       lid[d] = t % NP;
       t /= NP;
-      cid[d] = int(c % n);
+      cid[d] = c % n;
       c /= n;
     }
   }
@@ -358,7 +358,10 @@ k_sipg_lines(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
   const int64_t cell = int64_t(blockIdx.x) * CPB + csafe;
   const bool act = in && cell < ncell;
   const int64_t nn = int64_t(n) * n;
-  const int cx = int(cell % n), cy = int((cell / n) % n), cz = int(cell / nn);
+  // 32-bit coordinates (cells < n^3 <= 1000^3 < 2^31): 64-bit divisions by a
+  // run-time n are long instruction sequences
+  const int ci = int(cell);
+  const int cx = ci % n, cy = (ci / n) % n, cz = ci / (n * n);
   const int lofs = NP * ly + NL * lz;  // this line within the cell
   CT xo[NP];
   if(act)

@@ -144,11 +144,11 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
   const int ts = act ? t : 0;
   int cc[3] = {0, 0, 0};
   {
-    int64_t r = cell;
+    int r = int(cell);  // cells < n^3 <= 1000^3 < 2^31
 #pragma unroll
     for(int d = 0; d < DIM; ++d)
     {
-      cc[d] = int(r % n);
+      cc[d] = r % n;
       r /= n;
     }
   }
