@@ -89,6 +89,8 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
     return sl < count ? (cells ? int64_t(cells[sl]) : sl) : int64_t(-1);
   };
   const int64_t cell = act ? cell_of(cs) : 0;
+  // slot s of the block holds a cell (no load of the cell list)
+  auto has_cell = [&](int s) { return int64_t(blockIdx.x) * CPB + s < count; };
   const int ix = ts % NP, iy = (ts / NP) % NP, iz = DIM == 3 ? ts / (NP * NP) : 0;
   const int ti[3] = {ix, iy, iz};
 
@@ -318,7 +320,7 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
       const int s = o / (2 * 2 * DIM * NF), r = o % (2 * 2 * DIM * NF);
       const int side = r / (2 * DIM * NF), sd = (r / (DIM * NF)) % 2;
       const int c = (r / NF) % DIM, m = r % NF;
-      if(cell_of(s) < 0) continue;
+      if(!has_cell(s)) continue;
       const bool inner = nbc[s][2 * D + side] >= 0;
       if(sd == 1 && !inner) continue;
       const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
@@ -354,7 +356,7 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
     for(int o = t; o < CPB * 2 * NF; o += NT)
     {
       const int s = o / (2 * NF), side = (o / NF) % 2, l = o % NF;
-      if(cell_of(s) < 0) continue;
+      if(!has_cell(s)) continue;
       const bool inner = nbc[s][2 * D + side] >= 0;
       const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
       // traces and physical gradients of this cell (o) and the exterior state (x)
@@ -439,7 +441,7 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
     {
       const int s = o / (2 * DIM * NF), side = (o / (DIM * NF)) % 2;
       const int c = (o / NF) % DIM, m = o % NF;
-      if(cell_of(s) < 0) continue;
+      if(!has_cell(s)) continue;
       const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
       T pv = 0, pd = 0;
 #pragma unroll
