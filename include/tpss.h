@@ -194,7 +194,7 @@ int tpss_gmg_create(const tpss_gmg_desc *desc, tpss_gmg **out);
 int tpss_gmg_destroy(tpss_gmg *g);
 int tpss_gmg_level(tpss_gmg *g, int level, tpss_level **out);
 int tpss_gmg_smoother(tpss_gmg *g, int level, int post, tpss_smoother **out);
-/* one V-cycle dst = B src on the fine level (FP64 vectors) */
+/* one V-cycle dst = B src on the fine level (FP64 vectors; dst may equal src) */
 int tpss_gmg_vcycle(tpss_gmg *g, void *dst, const void *src, void *stream);
 /* GMG-preconditioned CG in FP64 (x, b: FP64 device vectors of the fine level) */
 int tpss_gmg_solve(tpss_gmg *g, void *x, const void *b, double rel_tol, int max_iter,
