@@ -159,8 +159,18 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   // matrices would need twice the scalar registers and spill.)
   constexpr bool WAVE_PATCH = FD == 0 && L == 64 && MS::THREADS == PPB * 64;
   constexpr bool S_SCALAR = WAVE_PATCH && std::is_same<CT, float>::value;
+  // one wave per patch with the matrices not in scalar registers (FP64): the
+  // even-odd products take the half matrices straight from scalar loads, so
+  // only patches without mirror symmetry stage the full matrices in LDS
+  bool stage_S = true;
+  if constexpr(WAVE_PATCH && !S_SCALAR)
+  {
+    const int jw = min(j0, patch_end - 1);
+    stage_S = !(eo_tab && (eo_tab[fac[size_t(jw) * DIM]] & eo_tab[fac[size_t(jw) * DIM + 1]] &
+                           eo_tab[fac[size_t(jw) * DIM + 2]]) != 0u);
+  }
   if constexpr(!S_SCALAR)
-    for(int idx = threadIdx.x; idx < PPB * DIM * K * K; idx += blockDim.x)
+    for(int idx = threadIdx.x; stage_S && idx < PPB * DIM * K * K; idx += blockDim.x)
     {
       const int q = idx / (DIM * K * K), r = idx % (DIM * K * K);
       const int d = r / (K * K), e = r % (K * K);
