@@ -102,7 +102,7 @@ __device__ __forceinline__ T invert(const T (*J)[3], T (*Ji)[3])
 }
 
 template<int DIM, int NP, typename VT, bool RESIDUAL>
-__global__ void __launch_bounds__((cpow(NP, DIM) <= 64 ? 64 : 128))
+__global__ void __launch_bounds__((cpow(NP, DIM) <= 64 ? 64 : 128), (std::is_same<VT, float>::value ? 5 : 1))
 k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
                const double *__restrict__ vert, const GeneralTab<NP, VT> tab, const int n,
                const double ip, const int p, const uint32_t *__restrict__ cells)
