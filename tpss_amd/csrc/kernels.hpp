@@ -1079,18 +1079,31 @@ __device__ __forceinline__ int fine_tensor_index(int child, int local)
 }
 
 template<int DIM>
-__device__ __forceinline__ int64_t child_cell(int64_t ccell, int nc, int child)
+__device__ __forceinline__ int64_t child_cell(const int *cc, int nc, int child)
 {
   int64_t f = 0, st = 1;
   const int nf = 2 * nc;
 #pragma unroll
   for(int d = 0; d < DIM; ++d)
   {
-    f += int64_t(2 * int(ccell % nc) + ((child >> d) & 1)) * st;
-    ccell /= nc;
+    f += int64_t(2 * cc[d] + ((child >> d) & 1)) * st;
     st *= nf;
   }
   return f;
+}
+
+// coordinates of coarse cell ccell (32-bit: cells < 1000^3 < 2^31; 64-bit
+// divisions by a run-time nc are long instruction sequences)
+template<int DIM>
+__device__ __forceinline__ void cell_coords(int64_t ccell, int nc, int *cc)
+{
+  int r = int(ccell);
+#pragma unroll
+  for(int d = 0; d < DIM; ++d)
+  {
+    cc[d] = r % nc;
+    r /= nc;
+  }
 }
 
 // xf += P xc
@@ -1104,6 +1117,8 @@ k_prolongate_add(T *__restrict__ xf, const T *__restrict__ xc, const TransferTab
   __shared__ T Pf[K * NP];
   __shared__ T S0[NFT], S1[NFT];
   const int64_t cc = blockIdx.x;
+  int ccd[3] = {0, 0, 0};
+  cell_coords<DIM>(cc, nc, ccd);
   const int t = threadIdx.x;
   for(int i = t; i < K * NP; i += NT) Pf[i] = T(tab.P[(i / NP) / NP][((i / NP) % NP) * NP + i % NP]);
   for(int i = t; i < NDC; i += NT) S0[i] = xc[cc * NDC + i];
@@ -1142,7 +1157,7 @@ k_prolongate_add(T *__restrict__ xf, const T *__restrict__ xc, const TransferTab
     T s = 0;
 #pragma unroll
     for(int j = 0; j < NP; ++j) s += Pf[Z * NP + j] * last[rest + cpow(K, DIM - 1) * j];
-    T *dst = xf + child_cell<DIM>(cc, nc, child) * NDC + local;
+    T *dst = xf + child_cell<DIM>(ccd, nc, child) * NDC + local;
     *dst = *dst + s;
   }
 }
@@ -1158,12 +1173,14 @@ k_restrict(T *__restrict__ xc, const T *__restrict__ xf, const TransferTab<NP> t
   __shared__ T Pf[K * NP];
   __shared__ T S0[NFT], S1[NFT];
   const int64_t cc = blockIdx.x;
+  int ccd[3] = {0, 0, 0};
+  cell_coords<DIM>(cc, nc, ccd);
   const int t = threadIdx.x;
   for(int i = t; i < K * NP; i += NT) Pf[i] = T(tab.P[(i / NP) / NP][((i / NP) % NP) * NP + i % NP]);
   for(int o = t; o < NFT; o += NT)
   {
     const int child = o / NDC, local = o % NDC;
-    S0[fine_tensor_index<DIM, NP>(child, local)] = xf[child_cell<DIM>(cc, nc, child) * NDC + local];
+    S0[fine_tensor_index<DIM, NP>(child, local)] = xf[child_cell<DIM>(ccd, nc, child) * NDC + local];
   }
   __syncthreads();
   // last direction: S1[rest + KL jl] = sum_L Pf[L][jl] S0[rest + KL L]
