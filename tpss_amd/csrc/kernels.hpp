@@ -855,10 +855,10 @@ k_avp_sum(const CT *__restrict__ ubuf, VT *__restrict__ dst, const int n, const 
   const int l = int(i % NDC);
   int c[3] = {0, 0, 0};
   {
-    int64_t t = cell;
+    int t = int(cell);  // cells < n^3 <= 1000^3 < 2^31
     for(int d = 0; d < DIM; ++d)
     {
-      c[d] = int(t % n);
+      c[d] = t % n;
       t /= n;
     }
   }
