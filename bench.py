@@ -6,10 +6,10 @@ DoFs/patch, 16 red-black colors), seeded U[-1,1] right-hand side (synthetic).
 
 A "step" is one SchwarzSmoother::step (Richardson: residual + one full
 multiplicative Schwarz sweep over all patches), inputs resident in HBM.
-value = patch-DoFs/s of the mixed-precision smoother (FP32 vectors and patch
-solves; the FP64 smoother is reported beside it), plus the GMG-preconditioned
-CG time to a 1e-8 relative residual for the FP64 and the mixed V-cycle
-(outer CG, residual and coarse solve in FP64).
+value = patch-DoFs/s of the FP64 smoother (the reference's arithmetic; the
+FP32 and BF16-MFMA smoothers are reported beside it), plus the
+GMG-preconditioned CG time to a 1e-8 relative residual for the FP64 and the
+mixed V-cycles (outer CG, residual and coarse solve in FP64).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -25,10 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
-# HBM bytes per launch of the FP32 fused color pass on the 64^3 Q3 level, from two
-# rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a copy kernel of
-# known size): profiles/mvp_fused_pass_64cubed_q3_hbm_traffic.md
-FUSED_PASS_TRAFFIC_BYTES = {(64, 3): 2.0 * 52113.6 * 1024 + 31255.9 * 1024}
+# measured HBM bytes per launch of the fused color pass (rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, each scaled by what the counter reads for a copy kernel of known size),
+# written by the profiling recipe in that file's header
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mvp_fused_pass_64cubed_q3_hbm_traffic.json")
 
 
 def parse():
@@ -51,38 +51,62 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run 5 FP64 -> FP32 conversions of N values (8N bytes read, 4N "
                          "written): the reference kernel for scaling HBM counter readings")
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-cores CPU run (0: the CPUs this process may use, "
+                         "capped by OMP_NUM_THREADS)")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the all-cores CPU run")
+    ap.add_argument("--cpu-steps-1core", type=int, default=1, help="steps of the 1-core CPU run")
     return ap.parse_args()
 
 
-def residual_kernel_bytes(sm, n, ndc, s_v):
-    """Algorithmic bytes of the localized residual r = b - A x on the cells of
-    each color (read x on the cells and their face neighbours, read b, write r),
-    averaged over the colors that get a residual update."""
+def color_pass_bytes(sm, n, ndc, s_v, first_reads_outer=True):
+    """Compulsory HBM bytes of one fused color pass, averaged over the color
+    sequence of a sweep: read b and write x on the cells of the color's
+    patches, read every outer face-neighbour cell of the color once (none in
+    the first pass of a sweep that starts from x = 0)."""
     V = sm.patch_vertices()
     cp = sm.color_ptr()
     dim = V.shape[1]
-    totals = []
-    for c in range(1, sm.n_colors):  # colors after the first need a residual
+    per_color = []
+    for c in range(sm.n_colors):
         vs = V[cp[c]:cp[c + 1]]
         cells = set()
         for off in np.ndindex(*([2] * dim)):
             cc = vs - 1 + np.array(off)
-            lin = cc[:, 0] + n * cc[:, 1] + (n * n * cc[:, 2] if dim == 3 else 0)
-            cells.update(lin.tolist())
-        cells = np.array(sorted(cells))
-        coords = np.stack([(cells // n ** d) % n for d in range(dim)], axis=1)
-        touched = set(cells.tolist())
+            cells.update((cc[:, 0] + n * cc[:, 1] + (n * n * cc[:, 2] if dim == 3 else 0)).tolist())
+        arr = np.array(sorted(cells))
+        coords = np.stack([(arr // n ** d) % n for d in range(dim)], axis=1)
+        outer = set()
         for d in range(dim):
-            for s in (-1, 1):
+            for sgn in (-1, 1):
                 nb = coords.copy()
-                nb[:, d] += s
+                nb[:, d] += sgn
                 ok = (nb[:, d] >= 0) & (nb[:, d] < n)
-                lin = sum(nb[ok, e] * n ** e for e in range(dim))
-                touched.update(lin.tolist())
-        totals.append((2 * len(cells) + len(touched)) * ndc * s_v)
-    return float(np.mean(totals))
+                outer.update(sum(nb[ok, e] * n ** e for e in range(dim)).tolist())
+        outer -= cells
+        per_color.append((2 * len(cells) * ndc * s_v, len(outer) * ndc * s_v))
+    seq = list(range(sm.n_colors))
+    total = sum(own + (outer if (k > 0 or first_reads_outer) else 0.0)
+                for k, (own, outer) in enumerate(per_color[c] for c in seq))
+    return total / len(seq)
+
+
+def host_cpus():
+    """(CPUs this process may use, CPUs of the host, CPU model name)"""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return usable, os.cpu_count() or usable, model
 
 
 def time_smoother(t, lv, sm, steps, warmup, precision):
@@ -177,17 +201,40 @@ def fit_section(t, n=48, p=4, n_root=3, amplitude=0.15):
     return out
 
 
-def cpu_baseline(n, p, steps, threads):
+def cpu_baseline(n, p, args):
+    """The C oracle (FP64 restatement of the reference smoother, full residual per
+    color, OpenMP over the patches of a color) on the same level, protocol of
+    apps/poisson_mpiscaling.cc:409-471 (setup timed apart, then timed steps on a
+    seeded vector) with a bounded sample: cpu_steps steps on all cores, then
+    cpu_steps_1core steps on one core, instead of 10 samples x 20 steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as o
-    sm = o.Smoother(3, p, n, variant=o.MULTIPLICATIVE, coloring=o.COLOR_REDBLACK, threads=threads)
-    b = o.splitmix_uniform(2, sm.N)
-    x = np.zeros(sm.N)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sm.step(x, b)
-    dt = (time.perf_counter() - t0) / steps
-    return sm.P * sm.ndp / dt, dt
+    usable, total, model = host_cpus()
+    threads = args.cpu_threads
+    if threads <= 0:
+        threads = usable
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    out = {}
+    for label, th, steps in (("all_cores", threads, args.cpu_steps),
+                             ("one_core", 1, args.cpu_steps_1core)):
+        t0 = time.perf_counter()
+        sm = o.Smoother(3, p, n, variant=o.MULTIPLICATIVE, coloring=o.COLOR_REDBLACK, threads=th)
+        setup = time.perf_counter() - t0
+        b = o.splitmix_uniform(2, sm.N)
+        x = np.zeros(sm.N)
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            sm.step(x, b)
+            times.append(time.perf_counter() - t0)
+        med = float(np.median(times))
+        out[label] = {"threads": th, "steps": steps, "setup_seconds": setup,
+                      "median_step_seconds": med, "min_step_seconds": float(min(times)),
+                      "patch_dofs_per_s": sm.P * sm.ndp / med}
+        del sm
+    out["host"] = {"usable_cpus": usable, "host_cpus": total, "model": model}
+    return out
 
 
 def main():
@@ -218,7 +265,7 @@ def main():
     wanted = set(args.variants.split(","))
     for name, prec, lprec in (("f64", t.F64, t.F64), ("f32", t.F32, t.F32),
                               ("bf16", t.F32, t.BF16)):
-        if name != "f32" and name not in wanted:  # the FP32 variant is the headline value
+        if name != "f64" and name not in wanted:  # the FP64 variant is the headline value
             continue
         lv = t.Level(dim, p, n, precision=prec)
         sm = t.SchwarzSmoother(lv, variant=t.MULTIPLICATIVE, coloring=t.COLOR_REDBLACK,
@@ -228,13 +275,15 @@ def main():
         pd = sm.n_patches * sm.n_dofs_per_patch
         apply_t, apply_n = sections["[SchwarzPrecond] Apply inverses:"]
         res_t, res_n = sections["[SchwarzPrecond] Update residuals:"]
-        # residual-free color pass: read b_j, the 24 outer neighbour cells (3 n_pd
-        # values) and write x_j per patch
-        patch_bytes = sm.n_patches / sm.n_colors * 5 * sm.n_dofs_per_patch * s_v
-        res_bytes = residual_kernel_bytes(sm, n, ndc, s_v)
+        # FP64/FP32: residual-free color passes on x (every pass reads the outer
+        # neighbour cells); BF16: Richardson residual b - A x (read x, b, write r:
+        # 3 N s_v), then color passes on the correction, which starts from zero
+        fused = name != "bf16"
+        patch_bytes = color_pass_bytes(sm, n, ndc, s_v, first_reads_outer=fused)
+        res_bytes = 3.0 * lv.n_dofs * s_v
         kernels = {
-            "patch_fd": {"avg_ms": 1e3 * apply_t / max(apply_n, 1), "launches": apply_n,
-                         "bytes": patch_bytes},
+            "color_pass": {"avg_ms": 1e3 * apply_t / max(apply_n, 1), "launches": apply_n,
+                           "bytes": patch_bytes},
             "residual": {"avg_ms": 1e3 * res_t / max(res_n, 1), "launches": res_n,
                          "bytes": res_bytes},
         }
@@ -299,15 +348,26 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        rate, dt = cpu_baseline(n, p, args.cpu_steps, args.cpu_threads)
-        cpu = {"value": rate, "unit": "patch-DoFs/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"{args.cpu_steps} FP64 MVP smoothing steps of the C oracle on the same "
-                         f"{n}^3 Q{p} level ({dt:.2f} s/step, full residual per color as the "
-                         f"reference)"}
+        cb = cpu_baseline(n, p, args)
+        a1, c1, hst = cb["all_cores"], cb["one_core"], cb["host"]
+        cpu = {"value": a1["patch_dofs_per_s"], "unit": "patch-DoFs/s", "cores": a1["threads"],
+               "kind": "port",
+               "sample": f"FP64 MVP smoothing steps (full residual per color, as the reference) of "
+                         f"the C oracle on the same {n}^3 Q{p} level: median of {a1['steps']} "
+                         f"steps on {a1['threads']} threads ({a1['median_step_seconds']:.2f} "
+                         f"s/step) and of {c1['steps']} on 1 core "
+                         f"({c1['median_step_seconds']:.2f} s/step); capped from 10 samples x "
+                         f"20 steps to keep the run within minutes",
+               "one_core": c1["patch_dofs_per_s"], "runs": cb,
+               "host": f"{hst['model']}, {hst['usable_cpus']} usable of {hst['host_cpus']} CPUs"}
 
-    main_v = results["f32"]
+    main_v = results["f64"]
     dom = main_v["dominant"]
     k = main_v["kernels"][dom]
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and dom == "color_pass":
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get(f"f64_{n}_{p}", {}).get("bytes_per_launch")
     out = {
         "metric": "smoother patch-DoFs/s + GMG-CG time-to-1e-8: mixed/learned vs FP64, 1 MI355X",
         "value": main_v["patch_dofs_per_s"],
@@ -319,15 +379,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic (seeded U[-1,1] right-hand side, Cartesian unit cube)",
         "config": {"workload": f"3D SIPG DG Q{p} Poisson {n}^3 cells, multiplicative vertex-patch "
-                               f"Schwarz smoother step (16 red-black colors, 512 DoFs/patch)",
+                               f"Schwarz smoother step (16 red-black colors, 512 DoFs/patch), "
+                               f"FP64 vectors and patch solves",
                    "n_dofs": main_v["n_dofs"], "n_patches": main_v["n_patches"],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": k["gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": k["gbs"] / HBM_PEAK_GBS,
-                     "traffic": FUSED_PASS_TRAFFIC_BYTES.get((n, p)) if dom == "patch_fd" else None},
+                     "algorithmic_bytes_per_launch": k["bytes"], "traffic": traffic},
         "cpu_baseline": cpu,
         "variants": {name: {"patch_dofs_per_s": r["patch_dofs_per_s"],
                             "ms_per_step": r["ms_per_step"],

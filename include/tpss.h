@@ -58,7 +58,10 @@ typedef struct tpss_gmg tpss_gmg;
 
 /* One multigrid level: mesh, DG Q_p space, SIPG Poisson operator.
  * Replaces Poisson::ModelProblem mesh/DoF setup (apps/poisson_problem.h:262-306)
- * + Laplace::DG::MF::Operator (apps/laplace_integrator.h:578-743). */
+ * + Laplace::DG::MF::Operator (apps/laplace_integrator.h:578-743).
+ * Limits: at most 2^24 cells (3D: n_cells_1d <= 255) and 2^31 DoFs per level
+ * (TPSS_E_INVALID_ARG beyond); with vertices, every cell must have a positive
+ * Jacobian determinant at each corner (TPSS_E_SINGULAR otherwise). */
 typedef struct
 {
   int dim;           /* 2 or 3 */
@@ -106,10 +109,11 @@ const char *tpss_version(void);
 int tpss_level_create(const tpss_level_desc *desc, tpss_level **out);
 int tpss_level_destroy(tpss_level *level);
 int tpss_level_n_dofs(const tpss_level *level, int64_t *n_dofs);
-/* dst = A src; Laplace::DG::MF::Operator::vmult (apps/laplace_integrator.h:578-594) */
+/* dst = A src; Laplace::DG::MF::Operator::vmult (apps/laplace_integrator.h:578-594);
+ * dst must not alias src */
 int tpss_operator_vmult(tpss_level *level, void *dst, const void *src, void *stream);
 /* dst = b - A x (the residual update sadd(-1, 1, rhs) after vmult,
- * schwarz_preconditioner.templates.h:721-725) */
+ * schwarz_preconditioner.templates.h:721-725); dst aliases neither b nor x */
 int tpss_operator_residual(tpss_level *level, void *dst, const void *b, const void *x,
                            void *stream);
 
@@ -124,8 +128,22 @@ int tpss_smoother_create(tpss_level *level, const tpss_smoother_desc *desc, tpss
 int tpss_smoother_clone_shallow(const tpss_smoother *other, const tpss_smoother_desc *desc,
                                 tpss_smoother **out);
 int tpss_smoother_destroy(tpss_smoother *sm);
-/* SchwarzPreconditioner::vmult (schwarz_preconditioner.templates.h:209-239); dst is overwritten */
+/* SchwarzPreconditioner::vmult (schwarz_preconditioner.templates.h:209-239); dst is overwritten.
+ * All apply entry points of a smoother require dst != src and first report
+ * TPSS_E_NONFINITE (doing nothing else) if the non-finite check has flagged
+ * the output of an earlier call (see tpss_smoother_set_check_finite). */
 int tpss_precond_vmult(tpss_smoother *sm, void *dst, const void *src, void *stream);
+/* SchwarzPreconditioner::vmult_add (schwarz_preconditioner.templates.h:187-194): dst += P src */
+int tpss_precond_vmult_add(tpss_smoother *sm, void *dst, const void *src, void *stream);
+/* SchwarzPreconditioner::Tvmult (schwarz_preconditioner.h:187, templates.h:243-268):
+ * dst = P^T src.  The reference declares it but throws ("transpose operation is
+ * not implemented", templates.h:672, 698) while its get_color_sequence(transpose)
+ * reverses the colors (schwarz_preconditioner.h:531-567).  Here: the
+ * multiplicative sweep over the reversed color sequence of vmult (the exact
+ * transpose, local solvers being symmetric); the additive operator is symmetric. */
+int tpss_precond_Tvmult(tpss_smoother *sm, void *dst, const void *src, void *stream);
+/* SchwarzPreconditioner::Tvmult_add (templates.h:197-206): dst += P^T src */
+int tpss_precond_Tvmult_add(tpss_smoother *sm, void *dst, const void *src, void *stream);
 /* SmootherRichardson::step (smoother_richardson.h:118-145) */
 int tpss_smoother_step(tpss_smoother *sm, void *dst, const void *src, void *stream);
 /* SmootherRichardson::vmult (smoother_richardson.h:147-156) */
@@ -154,9 +172,19 @@ int tpss_smoother_fit_errors(const tpss_smoother *sm, double *out);
  * "Copy Locally Owned", "Step"; *n in: capacity, out: count. */
 int tpss_smoother_get_timings(const tpss_smoother *sm, tpss_time_info *out, int *n);
 int tpss_smoother_set_timing(tpss_smoother *sm, int enabled);
-/* always-on AssertIsFinite (schwarz_preconditioner.templates.h:238 is debug-only
- * in the reference); costs one device synchronisation per call */
-int tpss_smoother_set_check_finite(tpss_smoother *sm, int enabled);
+/* AssertIsFinite(dst.l2_norm()) (schwarz_preconditioner.templates.h:238, debug-only
+ * in the reference), on by default: each apply entry point ends with one pass
+ * over dst that sets a flag in pinned host memory when it meets NaN/Inf.
+ * mode 1 (default): no host synchronisation; the flag is reported as
+ * TPSS_E_NONFINITE by the next apply entry point of this smoother or by
+ * tpss_smoother_finite_status.  mode 2: the call synchronises its stream and
+ * reports at once.  mode 0: off.  Smoothers inside tpss_gmg_solve are not
+ * checked per call: the CG's dot products reduce every entry, and a non-finite
+ * residual ends the solve with TPSS_E_NONFINITE. */
+int tpss_smoother_set_check_finite(tpss_smoother *sm, int mode);
+/* synchronises stream, then returns TPSS_E_NONFINITE (and clears the flag) if
+ * an output since the last report was non-finite, else TPSS_OK */
+int tpss_smoother_finite_status(tpss_smoother *sm, void *stream);
 /* Multiplicative variant on Cartesian meshes with red-black colors: each color
  * pass computes b_j - B_j x_out inside the patch kernel instead of a separate
  * residual update (default on; 0 selects the residual-update path that the

@@ -180,6 +180,19 @@ public:
   {
     check(tpss_precond_vmult(h_, dst.data(), src.data(), nullptr));
   }
+  // vmult_add / Tvmult / Tvmult_add (schwarz_preconditioner.h:185-194)
+  void vmult_add(VectorType &dst, const VectorType &src) const
+  {
+    check(tpss_precond_vmult_add(h_, dst.data(), src.data(), nullptr));
+  }
+  void Tvmult(VectorType &dst, const VectorType &src) const
+  {
+    check(tpss_precond_Tvmult(h_, dst.data(), src.data(), nullptr));
+  }
+  void Tvmult_add(VectorType &dst, const VectorType &src) const
+  {
+    check(tpss_precond_Tvmult_add(h_, dst.data(), src.data(), nullptr));
+  }
   std::vector<tpss_time_info> get_time_data() const
   {
     std::vector<tpss_time_info> t(8);

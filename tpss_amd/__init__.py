@@ -75,11 +75,13 @@ EXPORTS = [
     "tpss_last_error", "tpss_version", "tpss_level_create", "tpss_level_destroy",
     "tpss_level_n_dofs", "tpss_operator_vmult", "tpss_operator_residual",
     "tpss_smoother_create", "tpss_smoother_clone_shallow", "tpss_smoother_destroy",
-    "tpss_precond_vmult", "tpss_smoother_step", "tpss_smoother_vmult",
+    "tpss_precond_vmult", "tpss_precond_vmult_add", "tpss_precond_Tvmult",
+    "tpss_precond_Tvmult_add", "tpss_smoother_step", "tpss_smoother_vmult",
     "tpss_smoother_set_patch_damping", "tpss_smoother_info", "tpss_smoother_color_ptr",
     "tpss_smoother_patch_vertices", "tpss_smoother_patch_dof_indices", "tpss_smoother_patch_h",
     "tpss_smoother_fit_errors",
     "tpss_smoother_get_timings", "tpss_smoother_set_timing", "tpss_smoother_set_check_finite",
+    "tpss_smoother_finite_status",
     "tpss_smoother_set_fused_residual",
     "tpss_gmg_create", "tpss_gmg_destroy", "tpss_gmg_level", "tpss_gmg_smoother",
     "tpss_gmg_vcycle", "tpss_gmg_solve", "tpss_transfer_prolongate_add",
@@ -114,6 +116,9 @@ def _load():
         "tpss_smoother_clone_shallow": (ctypes.c_int, [_VP, ctypes.POINTER(SmootherDesc), _PVP]),
         "tpss_smoother_destroy": (ctypes.c_int, [_VP]),
         "tpss_precond_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_precond_vmult_add": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_precond_Tvmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_precond_Tvmult_add": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tpss_smoother_step": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tpss_smoother_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tpss_smoother_set_patch_damping": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
@@ -125,6 +130,7 @@ def _load():
         "tpss_smoother_get_timings": (ctypes.c_int, [_VP, ctypes.POINTER(TimeInfo), ip]),
         "tpss_smoother_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
         "tpss_smoother_set_check_finite": (ctypes.c_int, [_VP, ctypes.c_int]),
+        "tpss_smoother_finite_status": (ctypes.c_int, [_VP, _VP]),
         "tpss_smoother_set_fused_residual": (ctypes.c_int, [_VP, ctypes.c_int]),
         "tpss_gmg_create": (ctypes.c_int, [ctypes.POINTER(GMGDesc), _PVP]),
         "tpss_gmg_destroy": (ctypes.c_int, [_VP]),
@@ -297,8 +303,10 @@ class Level:
         self._owned = True
 
     @classmethod
-    def _borrow(cls, handle, dim, degree, n, precision):
+    def _borrow(cls, handle, dim, degree, n, precision, owner):
+        """wraps a level owned by ``owner`` (kept alive as long as the wrapper)"""
         self = cls.__new__(cls)
+        self._owner = owner
         self.handle = handle
         self.dim, self.degree, self.n, self.precision = dim, degree, n, precision
         n_dofs = ctypes.c_int64()
@@ -324,7 +332,7 @@ class SchwarzPreconditioner:
 
     def __init__(self, level, variant=ADDITIVE, coloring=None, relaxation=None, reverse=False,
                  symmetrized=False, damping_factor=1.0, n_steps=1, local_precision=None,
-                 local_solver=LOCAL_SURROGATE, fit_sweeps=0, _handle=None):
+                 local_solver=LOCAL_SURROGATE, fit_sweeps=0, _handle=None, _owner=None):
         if coloring is None:
             coloring = COLOR_NONE if variant == ADDITIVE else COLOR_REDBLACK
         if relaxation is None:
@@ -336,6 +344,8 @@ class SchwarzPreconditioner:
                                    damping_factor, n_steps, local_precision, local_solver,
                                    fit_sweeps)
         if _handle is not None:
+            # borrowed from _owner (a GMGSolver), which must outlive this wrapper
+            self._owner = _owner
             self.handle, self._owned = _handle, False
         else:
             h = ctypes.c_void_p()
@@ -364,6 +374,15 @@ class SchwarzPreconditioner:
 
     def vmult(self, dst, src, stream=None):
         _check(lib.tpss_precond_vmult(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def vmult_add(self, dst, src, stream=None):
+        _check(lib.tpss_precond_vmult_add(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def Tvmult(self, dst, src, stream=None):
+        _check(lib.tpss_precond_Tvmult(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def Tvmult_add(self, dst, src, stream=None):
+        _check(lib.tpss_precond_Tvmult_add(self.handle, _ptr(dst), _ptr(src), _stream(stream)))
 
     def set_patch_damping(self, omega):
         if omega is None:
@@ -399,8 +418,13 @@ class SchwarzPreconditioner:
     def set_fused_residual(self, enabled=True):
         _check(lib.tpss_smoother_set_fused_residual(self.handle, int(enabled)))
 
-    def set_check_finite(self, enabled=True):
-        _check(lib.tpss_smoother_set_check_finite(self.handle, int(enabled)))
+    def set_check_finite(self, mode=1):
+        """0 off; 1 on, reported at the next call (default); 2 on, synchronous"""
+        _check(lib.tpss_smoother_set_check_finite(self.handle, int(mode)))
+
+    def finite_status(self, stream=None):
+        """raises TPSSError(E_NONFINITE) if an output since the last report was non-finite"""
+        _check(lib.tpss_smoother_finite_status(self.handle, _stream(stream)))
 
     def get_time_data(self):
         n = ctypes.c_int(8)
@@ -495,7 +519,7 @@ class GMGSolver:
         h = ctypes.c_void_p()
         _check(lib.tpss_gmg_level(self.handle, l, ctypes.byref(h)))
         return Level._borrow(h, self.dim, self.degree, self.desc.n_cells_coarse << l,
-                             self.mg_precision)
+                             self.mg_precision, owner=self)
 
     def smoother(self, l, post=False):
         h = ctypes.c_void_p()
@@ -503,7 +527,8 @@ class GMGSolver:
         return SchwarzSmoother(self.level(l), _handle=h, variant=self.desc.smoother.variant,
                                coloring=self.desc.smoother.coloring,
                                relaxation=self.desc.smoother.relaxation,
-                               local_precision=self.desc.smoother.local_precision)
+                               local_precision=self.desc.smoother.local_precision,
+                               _owner=self)
 
     def vcycle(self, dst, src, stream=None):
         _check(lib.tpss_gmg_vcycle(self.handle, _ptr(dst), _ptr(src), _stream(stream)))

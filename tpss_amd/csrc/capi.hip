@@ -252,6 +252,53 @@ void convert(void *dst, int dp, const void *src, int sp, int64_t n, hipStream_t 
   TPSS_HIP(hipGetLastError());
 }
 
+// Rejects collapsed or inverted cells of a moved-vertex mesh: the Jacobian
+// determinant of the multilinear map must be positive at every corner of
+// every cell (host, before anything is allocated on the device).
+void check_cells(int dim, int n, const double *V)
+{
+  const int64_t ncell = ipow(n, dim);
+  for(int64_t c = 0; c < ncell; ++c)
+  {
+    int cc[3] = {0, 0, 0};
+    int64_t r = c;
+    for(int d = 0; d < dim; ++d)
+    {
+      cc[d] = int(r % n);
+      r /= n;
+    }
+    auto vtx = [&](int corner, int a) {
+      int64_t idx = 0, st = 1;
+      for(int d = 0; d < dim; ++d)
+      {
+        idx += int64_t(cc[d] + ((corner >> d) & 1)) * st;
+        st *= n + 1;
+      }
+      return V[idx * dim + a];
+    };
+    for(int corner = 0; corner < (1 << dim); ++corner)
+    {
+      // J[a][b] = dx_a / dxhat_b at the corner: edge from the corner along b
+      double J[3][3];
+      for(int b = 0; b < dim; ++b)
+      {
+        const int other = corner ^ (1 << b);
+        const double sgn = ((corner >> b) & 1) ? -1.0 : 1.0;
+        for(int a = 0; a < dim; ++a) J[a][b] = sgn * (vtx(other, a) - vtx(corner, a));
+      }
+      const double det =
+        dim == 2 ? J[0][0] * J[1][1] - J[0][1] * J[1][0]
+                 : J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      if(!(det > 0.0))
+        throw Error(TPSS_E_SINGULAR, "cell " + std::to_string(c) +
+                                       " is collapsed or inverted (Jacobian determinant " +
+                                       std::to_string(det) + " at a corner)");
+    }
+  }
+}
+
 }  // namespace tpss
 
 using namespace tpss;
@@ -285,6 +332,17 @@ int tpss_level_create(const tpss_level_desc *desc, tpss_level **out)
     lv->np = desc->degree + 1;
     lv->n = desc->n_cells_1d;
     lv->ndc = ipow(lv->np, lv->dim);
+    {
+      // every launch of this library keeps gridDim.x * blockDim.x below 2^32
+      // while a level has fewer than 2^32 / 256 cells and patches (one block of
+      // at most 256 threads per cell or patch) and fewer than 2^31 DoFs
+      // (32-bit DoF and cell indices in the index tables and cell lists)
+      const int64_t cells = int64_t(ipow(lv->n, lv->dim));
+      const int64_t dofs = cells * lv->ndc * (elasticity ? lv->dim : 1);
+      TPSS_REQUIRE(cells * 256 < (int64_t(1) << 32) && dofs < (int64_t(1) << 31),
+                   "level too large: at most 2^24 cells (3D: n_cells_1d <= 255) and 2^31 DoFs");
+    }
+    if(desc->vertices) check_cells(lv->dim, lv->n, desc->vertices);
     lv->ncell = ipow(lv->n, lv->dim);
     lv->Ns = lv->ncell * lv->ndc;
     lv->ncomp = elasticity ? lv->dim : 1;
@@ -324,6 +382,8 @@ int tpss_operator_vmult(tpss_level *level, void *dst, const void *src, void *str
 {
   return guarded([&] {
     TPSS_REQUIRE(level && dst && src, "null argument");
+    // the kernels read face-neighbour cells of src while writing dst
+    TPSS_REQUIRE(dst != src, "dst and src must not alias");
     operator_apply_any(level, dst, src, nullptr, nullptr, 0, as_stream(stream));
   });
 }
@@ -332,6 +392,7 @@ int tpss_operator_residual(tpss_level *level, void *dst, const void *b, const vo
 {
   return guarded([&] {
     TPSS_REQUIRE(level && dst && b && x, "null argument");
+    TPSS_REQUIRE(dst != x && dst != b, "dst must alias neither b nor x");
     operator_apply_any(level, dst, x, b, nullptr, 0, as_stream(stream));
   });
 }

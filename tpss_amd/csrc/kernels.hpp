@@ -961,12 +961,16 @@ k_dot_final(const double *__restrict__ partial, const int np, double *__restrict
   if(threadIdx.x == 0) *out = sh[0];
 }
 
-// non-finite detector: flag[0] = 1 if any entry is NaN/Inf
+// non-finite detector (grid-stride): flag[0] = 1 if any entry is NaN/Inf;
+// flag may point to pinned host memory
 template<typename T>
-__global__ void k_nonfinite(const T *__restrict__ x, const int64_t n, int *__restrict__ flag)
+__global__ void k_nonfinite(const T *__restrict__ x, const int64_t n, int *flag)
 {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if(i < n && !isfinite(double(x[i]))) flag[0] = 1;
+  bool bad = false;
+  for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+      i += int64_t(gridDim.x) * blockDim.x)
+    bad |= !isfinite(x[i]);
+  if(bad) *flag = 1;
 }
 
 // sum of one value per thread over a block of 256 threads (the tree of

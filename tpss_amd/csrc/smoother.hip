@@ -572,7 +572,7 @@ void resolve_timings(tpss_smoother *sm)
 // dst = P src  (or, if accumulate: dst += alpha * P src for the additive variant)
 template<typename VT, typename CT>
 void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
-                     double alpha)
+                     double alpha, const std::vector<int> &seq)
 {
   LocalSolvers &ls = *sm->ls;
   const tpss_level *lv = sm->level;
@@ -602,9 +602,9 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
   {
     // cells not yet reached by a color are read as neighbours: start from zero
     fill_zero(out, lv->vec_prec, lv->N, s);
-    for(size_t k = 0; k < sm->seq.size(); ++k)
+    for(size_t k = 0; k < seq.size(); ++k)
     {
-      const int c = sm->seq[k];
+      const int c = seq[k];
       SectionTimer t(sm, 1, s);
       mvp_fused_pass<VT, CT>(sm, src, out, ls.color_ptr[c], ls.color_ptr[c + 1], k == 0, s);
     }
@@ -614,9 +614,9 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
   sm->res.resize(size_t(lv->N) * sizeof(VT));
   VT *res = (VT *)sm->res.ptr;
   fill_zero(out, lv->vec_prec, lv->N, s);
-  for(size_t k = 0; k < sm->seq.size(); ++k)
+  for(size_t k = 0; k < seq.size(); ++k)
   {
-    const int c = sm->seq[k];
+    const int c = seq[k];
     const VT *r = src;
     if(k > 0)
     {
@@ -635,7 +635,7 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
 // Cartesian meshes with red-black colors, 3D Q3; vectors FP32 or FP64
 template<typename VT>
 void precond_vmult_bf16(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
-                        double alpha)
+                        double alpha, const std::vector<int> &seq)
 {
   LocalSolvers &ls = *sm->ls;
   const tpss_level *lv = sm->level;
@@ -651,41 +651,45 @@ void precond_vmult_bf16(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s
     out = (VT *)sm->tmp2.ptr;
   }
   fill_zero(out, lv->vec_prec, lv->N, s);
-  for(size_t k = 0; k < sm->seq.size(); ++k)
+  for(size_t k = 0; k < seq.size(); ++k)
   {
-    const int c = sm->seq[k];
+    const int c = seq[k];
     SectionTimer t(sm, 1, s);
     mvp_fused_pass<VT, float, 1>(sm, src, out, ls.color_ptr[c], ls.color_ptr[c + 1], k == 0, s);
   }
   if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
 }
 template void precond_vmult_bf16<double>(tpss_smoother *, double *, const double *, hipStream_t,
-                                         bool, double);
+                                         bool, double, const std::vector<int> &);
 template void precond_vmult_bf16<float>(tpss_smoother *, float *, const float *, hipStream_t, bool,
-                                        double);
+                                        double, const std::vector<int> &);
 
 template<typename VT>
 void precond_dispatch(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
-                      double alpha)
+                      double alpha, const std::vector<int> &seq)
 {
   const int lp = sm->desc.local_precision;
   if(std::is_same<VT, double>::value && lp == TPSS_F64)
-    precond_vmult_t<VT, double>(sm, dst, src, s, accumulate, alpha);
+    precond_vmult_t<VT, double>(sm, dst, src, s, accumulate, alpha, seq);
   else if(lp == TPSS_F32)
-    precond_vmult_t<VT, float>(sm, dst, src, s, accumulate, alpha);
+    precond_vmult_t<VT, float>(sm, dst, src, s, accumulate, alpha, seq);
   else if(lp == TPSS_BF16)
-    precond_vmult_bf16(sm, dst, src, s, accumulate, alpha);
+    precond_vmult_bf16(sm, dst, src, s, accumulate, alpha, seq);
   else
     throw Error(TPSS_E_UNSUPPORTED, "local precision F64 needs F64 level vectors");
 }
 
+// transpose: the additive operator is symmetric (symmetric local solvers,
+// scalar weights); the transpose of a multiplicative sweep is the sweep over
+// the reversed color sequence
 void precond_vmult_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
-                       bool accumulate, double alpha)
+                       bool accumulate, double alpha, bool transpose)
 {
+  const std::vector<int> &seq = transpose ? sm->seq_t : sm->seq;
   if(sm->level->vec_prec == TPSS_F64)
-    precond_dispatch<double>(sm, (double *)dst, (const double *)src, s, accumulate, alpha);
+    precond_dispatch<double>(sm, (double *)dst, (const double *)src, s, accumulate, alpha, seq);
   else
-    precond_dispatch<float>(sm, (float *)dst, (const float *)src, s, accumulate, alpha);
+    precond_dispatch<float>(sm, (float *)dst, (const float *)src, s, accumulate, alpha, seq);
 }
 
 // True if a Richardson step with omega_R = 1 can run as residual-free color
@@ -746,28 +750,48 @@ void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_
   sm->rich_r.resize(bytes);
   for(int k = 0; k < sm->desc.n_steps; ++k)
   {
-    operator_apply_any(lv, sm->rich_r.ptr, dst, src, nullptr, 0, s);
+    {
+      // the Richardson residual b - A x is timed with the residual updates
+      SectionTimer t(sm, 0, s);
+      operator_apply_any(lv, sm->rich_r.ptr, dst, src, nullptr, 0, s);
+    }
     precond_vmult_any(sm, dst, sm->rich_r.ptr, s, true, sm->desc.damping_factor);
   }
 }
 
+// throws (once) if a check kernel of an earlier call has flagged its output
+void report_nonfinite(tpss_smoother *sm)
+{
+  if(sm->nonfinite.host && __atomic_load_n(sm->nonfinite.host, __ATOMIC_ACQUIRE))
+  {
+    __atomic_store_n(sm->nonfinite.host, 0, __ATOMIC_RELEASE);
+    throw Error(TPSS_E_NONFINITE, "non-finite entries in the output of this smoother");
+  }
+}
+
+// AssertIsFinite(dst.l2_norm()) (schwarz_preconditioner.templates.h:238), on
+// always by default: one pass over the output that sets a pinned host flag
+// when it meets a NaN or Inf.  Mode 1 reports the flag at the next entry
+// point of this smoother (or tpss_smoother_finite_status) without a host
+// synchronisation here; mode 2 synchronises and reports at once.
 void check_finite(tpss_smoother *sm, const void *x, hipStream_t s)
 {
   if(!sm->check_finite) return;
   const tpss_level *lv = sm->level;
-  sm->flag.resize(1);
-  TPSS_HIP(hipMemsetAsync(sm->flag.ptr, 0, sizeof(int), s));
+  sm->nonfinite.ensure();
+  const unsigned grid = unsigned(std::min<int64_t>(2048, (lv->N + 255) / 256));
   if(lv->vec_prec == TPSS_F64)
-    hipLaunchKernelGGL((k_nonfinite<double>), dim3(blocks_for(lv->N, 256)), dim3(256), 0, s,
-                       (const double *)x, lv->N, sm->flag.ptr);
+    hipLaunchKernelGGL((k_nonfinite<double>), dim3(grid), dim3(256), 0, s, (const double *)x,
+                       lv->N, sm->nonfinite.dev);
   else
-    hipLaunchKernelGGL((k_nonfinite<float>), dim3(blocks_for(lv->N, 256)), dim3(256), 0, s,
-                       (const float *)x, lv->N, sm->flag.ptr);
+    hipLaunchKernelGGL((k_nonfinite<float>), dim3(grid), dim3(256), 0, s, (const float *)x,
+                       lv->N, sm->nonfinite.dev);
   TPSS_HIP(hipGetLastError());
-  int f = 0;
-  TPSS_HIP(hipMemcpyAsync(&f, sm->flag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-  TPSS_HIP(hipStreamSynchronize(s));
-  if(f) throw Error(TPSS_E_NONFINITE, "non-finite entries in the smoother output");
+  if(sm->check_finite == 2)
+  {
+    TPSS_HIP(hipStreamSynchronize(s));
+    report_nonfinite(sm);
+  }
 }
 
 std::vector<int> color_sequence(int n_colors, bool reverse, bool symmetrized)
@@ -821,6 +845,7 @@ int tpss_smoother_create(tpss_level *level, const tpss_smoother_desc *desc, tpss
     sm->desc = *desc;
     sm->ls = build_local_solvers(level, *desc, nullptr);
     sm->seq = color_sequence(sm->ls->n_colors, desc->reverse != 0, desc->symmetrized != 0);
+    sm->seq_t.assign(sm->seq.rbegin(), sm->seq.rend());
     sm->timers[2].time = sm->ls->setup_seconds;
     sm->timers[2].n_calls = 1;
     *out = sm.release();
@@ -840,6 +865,8 @@ int tpss_smoother_clone_shallow(const tpss_smoother *other, const tpss_smoother_
     sm->desc = *desc;
     sm->ls = other->ls;
     sm->seq = color_sequence(sm->ls->n_colors, desc->reverse != 0, desc->symmetrized != 0);
+    sm->seq_t.assign(sm->seq.rbegin(), sm->seq.rend());
+    sm->check_finite = other->check_finite;
     if(other->use_omega)
     {
       sm->d_omega.resize(other->d_omega.n);
@@ -863,7 +890,41 @@ int tpss_precond_vmult(tpss_smoother *sm, void *dst, const void *src, void *stre
   return guarded([&] {
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    report_nonfinite(sm);
     precond_vmult_any(sm, dst, src, as_stream(stream), false, 1.0);
+    check_finite(sm, dst, as_stream(stream));
+  });
+}
+
+int tpss_precond_vmult_add(tpss_smoother *sm, void *dst, const void *src, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && dst && src, "null argument");
+    TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    report_nonfinite(sm);
+    precond_vmult_any(sm, dst, src, as_stream(stream), true, 1.0);
+    check_finite(sm, dst, as_stream(stream));
+  });
+}
+
+int tpss_precond_Tvmult(tpss_smoother *sm, void *dst, const void *src, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && dst && src, "null argument");
+    TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    report_nonfinite(sm);
+    precond_vmult_any(sm, dst, src, as_stream(stream), false, 1.0, true);
+    check_finite(sm, dst, as_stream(stream));
+  });
+}
+
+int tpss_precond_Tvmult_add(tpss_smoother *sm, void *dst, const void *src, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && dst && src, "null argument");
+    TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    report_nonfinite(sm);
+    precond_vmult_any(sm, dst, src, as_stream(stream), true, 1.0, true);
     check_finite(sm, dst, as_stream(stream));
   });
 }
@@ -873,6 +934,7 @@ int tpss_smoother_step(tpss_smoother *sm, void *dst, const void *src, void *stre
   return guarded([&] {
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    report_nonfinite(sm);
     smoother_step_any(sm, dst, src, as_stream(stream));
     check_finite(sm, dst, as_stream(stream));
   });
@@ -883,6 +945,7 @@ int tpss_smoother_vmult(tpss_smoother *sm, void *dst, const void *src, void *str
   return guarded([&] {
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
+    report_nonfinite(sm);
     fill_zero(dst, sm->level->vec_prec, sm->level->N, as_stream(stream));
     smoother_step_any(sm, dst, src, as_stream(stream), true);
     check_finite(sm, dst, as_stream(stream));
@@ -1002,11 +1065,21 @@ int tpss_smoother_set_fused_residual(tpss_smoother *sm, int enabled)
   });
 }
 
-int tpss_smoother_set_check_finite(tpss_smoother *sm, int enabled)
+int tpss_smoother_set_check_finite(tpss_smoother *sm, int mode)
 {
   return guarded([&] {
     TPSS_REQUIRE(sm, "null argument");
-    sm->check_finite = enabled != 0;
+    TPSS_REQUIRE(mode >= 0 && mode <= 2, "check mode must be 0, 1 or 2");
+    sm->check_finite = mode;
+  });
+}
+
+int tpss_smoother_finite_status(tpss_smoother *sm, void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm, "null argument");
+    TPSS_HIP(hipStreamSynchronize(as_stream(stream)));
+    report_nonfinite(sm);
   });
 }
 

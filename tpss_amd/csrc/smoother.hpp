@@ -39,6 +39,32 @@ struct LocalSolvers
 
 void patch_dof_indices_device(const LocalSolvers &ls, uint32_t *host_out);
 
+// One int in pinned host memory that kernels set through its device address:
+// the non-finite check reports without a host synchronisation.
+struct PinnedFlag
+{
+  int *host = nullptr;
+  int *dev = nullptr;
+  PinnedFlag() = default;
+  PinnedFlag(const PinnedFlag &) = delete;
+  PinnedFlag &operator=(const PinnedFlag &) = delete;
+  ~PinnedFlag()
+  {
+    if(host) (void)hipHostFree(host);
+  }
+  void ensure()
+  {
+    if(host) return;
+    void *h = nullptr;
+    TPSS_HIP(hipHostMalloc(&h, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    host = static_cast<int *>(h);
+    *host = 0;
+    void *d = nullptr;
+    TPSS_HIP(hipHostGetDevicePointer(&d, h, 0));
+    dev = static_cast<int *>(d);
+  }
+};
+
 }  // namespace tpss
 
 struct tpss_smoother
@@ -46,11 +72,16 @@ struct tpss_smoother
   tpss_level *level = nullptr;
   tpss_smoother_desc desc{};
   std::shared_ptr<tpss::LocalSolvers> ls;
-  std::vector<int> seq;
+  std::vector<int> seq;    // color sequence of vmult (schwarz_preconditioner.h:531-567)
+  std::vector<int> seq_t;  // of Tvmult: seq reversed
   tpss::DeviceArray<float> d_omega;
   bool use_omega = false;
   bool timing = false;
-  bool check_finite = false;
+  // non-finite check of every output: 0 off, 1 on and reported at the next
+  // entry point or by tpss_smoother_finite_status (default), 2 on with a
+  // synchronisation in the call itself
+  int check_finite = 1;
+  tpss::PinnedFlag nonfinite;
   bool fused_mvp = true;  // residual-free color passes where the mesh allows them
   tpss::Timer timers[5];
   struct PendingSection
@@ -61,16 +92,15 @@ struct tpss_smoother
   std::vector<PendingSection> pending;
   // scratch (bytes)
   tpss::DeviceArray<char> res, tmp2, rich_r, ubuf;
-  tpss::DeviceArray<int> flag;
 };
 
 namespace tpss
 {
 void precond_vmult_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
-                       bool accumulate, double alpha);
+                       bool accumulate, double alpha, bool transpose = false);
 void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
                        bool x_is_zero = false);
 template<typename VT>
 void precond_vmult_bf16(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
-                        double alpha);
+                        double alpha, const std::vector<int> &seq);
 }  // namespace tpss
