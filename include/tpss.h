@@ -51,6 +51,13 @@ typedef enum { TPSS_POISSON = 0, TPSS_ELASTICITY = 1 } tpss_equation;
  * rank-2 Kronecker projection per direction) of the exact local matrix
  * R_j A R_j^T, initialised with the surrogate matrices (kroneckersvd.h:215-585). */
 typedef enum { TPSS_LOCAL_SURROGATE = 0, TPSS_LOCAL_FIT = 1 } tpss_local_solver;
+/* Restricted additive Schwarz.  TPSS_RAS_WEIGHTS: each local solution is
+ * divided by the number of patches containing the DoF before scatter_add
+ * (compute_ras_weights, schwarz_preconditioner.templates.h:126-165, :636-641).
+ * TPSS_RAS_BOOLEAN: restricted prolongation rscatter_add, each DoF taken from
+ * one patch only (patch_transfer.templates.h:140-158, restricted flags
+ * dof_info.templates.h:192-313). */
+typedef enum { TPSS_RAS_NONE = 0, TPSS_RAS_WEIGHTS = 1, TPSS_RAS_BOOLEAN = 2 } tpss_ras;
 
 typedef struct tpss_level tpss_level;
 typedef struct tpss_smoother tpss_smoother;
@@ -92,6 +99,10 @@ typedef struct
   int local_precision;   /* tpss_precision of the patch solves */
   int local_solver;      /* tpss_local_solver (0 = surrogate lengths) */
   int fit_sweeps;        /* ALS sweeps of TPSS_LOCAL_FIT (0 = 4) */
+  int ras;               /* restricted additive Schwarz (additive variant, Poisson):
+                            0 off, TPSS_RAS_WEIGHTS = use_ras_weights,
+                            TPSS_RAS_BOOLEAN = use_ras_boolean_weights
+                            (schwarz_preconditioner.h:501-502) */
 } tpss_smoother_desc;
 
 typedef struct
@@ -161,6 +172,10 @@ int tpss_smoother_patch_vertices(const tpss_smoother *sm, int *out);
  * scatter index table as the device kernels compute it, host
  * uint32[n_patches * n_dofs_per_patch] */
 int tpss_smoother_patch_dof_indices(const tpss_smoother *sm, uint32_t *out);
+/* DoFInfo::restricted_dof_flags_patchwise (dof_info.templates.h:192-313) as the
+ * device computes them: host uint8[n_patches * n_dofs_per_patch], 1 where the
+ * restricted prolongation of the patch skips the DoF (scalar levels) */
+int tpss_smoother_restricted_dof_flags(const tpss_smoother *sm, uint8_t *out);
 /* surrogate lengths h[patch][direction][cell_no_1d] (mapping_info.h:126-130), host */
 int tpss_smoother_patch_h(const tpss_smoother *sm, double *out);
 /* relative Frobenius error ||A_j - fit_j|| / ||A_j|| of every patch's separable
@@ -264,6 +279,43 @@ int tpss_smoother_set_damping_mlp(tpss_smoother *sm, tpss_mlp *m);
  * sigma: batch x rank (device). */
 int tpss_ksvd_batched(const double *A, int batch, int m1, int n1, int m0, int n0, int rank,
                       double *B1, double *B0, double *sigma, void *stream);
+/* compute_ksvd of a sum of Kronecker products (kroneckersvd.h:306-402):
+ * A = sum_{i < in_rank} Abig_i (x) Bsmall_i, Abig: batch x in_rank x m1 x n1,
+ * Bsmall: batch x in_rank x m0 x n0 (device, row-major); the Lanczos products use
+ * the terms (A is never formed); outputs as tpss_ksvd_batched. in_rank <= 16. */
+int tpss_ksvd_sum_batched(const double *Abig, const double *Bsmall, int batch, int in_rank, int m1,
+                          int n1, int m0, int n0, int rank, double *B1, double *B0, double *sigma,
+                          void *stream);
+/* compute_kcp (kroneckersvd.h:501-585): rank-`rank` CP decomposition by
+ * alternating least squares of sum_{i < in_rank} C_i (x) B_i (x) A_i, initialised
+ * with the first `rank` input terms, `iterations` sweeps (0: rank^2 + 10).
+ * in: batch x in_rank x [C_i | B_i | A_i] (row-major, sizes m[k] x n[k] of the
+ * k-th matrix C, B, A); out: batch x rank x [C | B | A].  rank <= in_rank <= 8,
+ * matrices of at most 144 entries. */
+int tpss_kcp_batched(const double *in, int batch, int in_rank, const int *m, const int *n,
+                     int rank, int iterations, double *out, void *stream);
+
+/* ---- generic Kronecker-sum matrices --------------------------------------- */
+/* Tensors::TensorProductMatrix<order> in the basic state (tensor_product_matrix.h:
+ * 136-304): A = sum_{r < rank} A_{r,order-1} (x) ... (x) A_{r,0} (dimension 0
+ * rightmost; vector index sum_d i_d prod_{e<d} size_e), A_{r,d} of size
+ * m[d] x n[d], order 1..3.  factors (device): batch x rank x order matrices,
+ * row-major, d = 0 first.  vmult / Tvmult / vmult_add
+ * (tensor_product_matrix.templates.h:123-291): dst = (add ? dst : 0) + A src
+ * (transpose: A^T src), src and dst batch x (prod n | prod m) (swapped for
+ * transpose); intermediate tensors of at most 4096 entries. */
+int tpss_kron_vmult(int order, int rank, const int *m, const int *n, int batch,
+                    const double *factors, const double *src, double *dst, int transpose, int add,
+                    void *stream);
+/* dense matrices (as_table / as_transpose_table): batch x M x N (or N x M) */
+int tpss_kron_table(int order, int rank, const int *m, const int *n, int batch,
+                    const double *factors, int transpose, double *out, void *stream);
+/* InverseTable (matrix_helper.h:21-68): SVD pseudo-inverse of a batch of n x n
+ * matrices (device, row-major), singular values s_i < threshold * s_max dropped
+ * (LAPACKFullMatrix::compute_inverse_svd), or with kernel_size >= 0 the
+ * kernel_size smallest ones (compute_inverse_svd_with_kernel); n <= 256 */
+int tpss_kron_pinv(const double *A, int n, int batch, double threshold, int kernel_size,
+                   double *out, void *stream);
 
 /* ---- grid transfer (MGTransferMatrixFree for FE_DGQ) --------------------- */
 int tpss_transfer_prolongate_add(tpss_level *coarse, tpss_level *fine, void *dst_fine,

@@ -135,13 +135,18 @@ struct AdditionalData
   int local_precision = -1;  // -1: the level's vector precision
   int local_solver = TPSS_LOCAL_SURROGATE;
   int fit_sweeps = 0;
+  // use_ras_weights / use_ras_boolean_weights (schwarz_preconditioner.h:501-502)
+  bool use_ras_weights = false;
+  bool use_ras_boolean_weights = false;
 
   tpss_smoother_desc desc(int vec_prec) const
   {
     return tpss_smoother_desc{variant,       coloring,       relaxation,
                               int(reverse),  int(symmetrized), damping_factor,
                               n_steps,       local_precision < 0 ? vec_prec : local_precision,
-                              local_solver,  fit_sweeps};
+                              local_solver,  fit_sweeps,
+                              use_ras_weights ? TPSS_RAS_WEIGHTS
+                                              : use_ras_boolean_weights ? TPSS_RAS_BOOLEAN : 0};
   }
 };
 

@@ -15,6 +15,7 @@ _LIB = os.path.join(_HERE, "libtpss_oracle.so")
 ADDITIVE, MULTIPLICATIVE = 0, 1
 COLOR_NONE, COLOR_TILED, COLOR_REDBLACK = 0, 1, 2
 LOCAL_SURROGATE, LOCAL_FIT = 0, 1
+RAS_NONE, RAS_WEIGHTS, RAS_BOOLEAN = 0, 1, 2
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -52,6 +53,9 @@ def _load():
         "or_n_components": (ctypes.c_int, [ctypes.c_void_p]),
         "or_set_local_solver": (None, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
         "or_fit_errors": (None, [ctypes.c_void_p, _D]),
+        "or_set_ras": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "or_ras_weights": (None, [ctypes.c_void_p, _D]),
+        "or_restricted_flags": (None, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ubyte)]),
         "or_fit_fallbacks": (ctypes.c_int, [ctypes.c_void_p]),
         "or_patch_local_matrix": (None, [ctypes.c_void_p, ctypes.c_int, _D]),
         "or_separable_fit": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, _D, _D, _D, ctypes.c_int]),
@@ -219,7 +223,7 @@ class Smoother:
     def __init__(self, dim, p, n, ip=1.0, variant=ADDITIVE, coloring=COLOR_NONE,
                  relaxation=1.0, reverse=False, symmetrized=False, damping=1.0,
                  n_steps=1, vertices=None, threads=1, elasticity=None, local_solver=LOCAL_SURROGATE,
-                 fit_sweeps=4):
+                 fit_sweeps=4, ras=0):
         """elasticity=(mu, lambda) selects the linear elasticity operator with
         dim components and block-diagonal local solvers."""
         self.dim, self.p, self.n = dim, p, n
@@ -249,6 +253,8 @@ class Smoother:
         self.k = 2 * (p + 1)
         self.ncomp = lib.or_n_components(self.h)
         self.ndp = self.ncomp * self.k ** dim  # patch-DoFs over all components
+        if ras and lib.or_set_ras(self.h, int(ras)):
+            raise ValueError("restricted additive Schwarz: additive, scalar problems only")
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -287,6 +293,17 @@ class Smoother:
     def fit_errors(self):
         out = np.zeros(self.P)
         lib.or_fit_errors(self.h, dp(out))
+        return out
+
+    def ras_weights(self):
+        out = np.zeros(self.N)
+        lib.or_ras_weights(self.h, dp(out))
+        return out
+
+    def restricted_flags(self):
+        """(P, ndp) uint8: 1 where rscatter_add skips the patch DoF"""
+        out = np.zeros((self.P, self.ndp), dtype=np.uint8)
+        lib.or_restricted_flags(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
         return out
 
     def fit_fallbacks(self):

@@ -475,6 +475,11 @@ struct or_ctx
   /* local solvers: surrogate-length FD (0) or separable fit of the exact local matrix (1) */
   int local_solver, fit_sweeps, n_fit_fallback;
   double *fit_err; /* P: relative Frobenius error of the fitted separable matrix */
+  /* restricted additive Schwarz: 0 off, 1 use_ras_weights, 2 use_ras_boolean_weights
+   * (schwarz_preconditioner.h:501-502) */
+  int ras;
+  double *ras_w;               /* N: number of patches containing each DoF */
+  unsigned char *ras_flags;    /* P * ndp: 1 = restricted (not scattered) */
   int ready;
 };
 
@@ -585,6 +590,8 @@ void or_destroy(or_ctx *c)
 {
   if(!c) return;
   free(c->fit_err);
+  free(c->ras_w);
+  free(c->ras_flags);
   free(c->vertices);
   free(c->color_ptr);
   free(c->vtx);
@@ -1159,12 +1166,100 @@ static void apply_local_solvers(const or_ctx *c, double *solution, const double 
       const double w = c->omega ? c->omega[j] : c->relaxation;
       if(w != 1.0)
         for(int i = 0; i < ndp; ++i) ul[i] *= w;
-      for(int i = 0; i < ndp; ++i) solution[idx[i]] += ul[i];
+      /* weighted RAS (:636-641) or restricted scatter rscatter_add (:646-649,
+       * patch_transfer.templates.h:140-158, scatter_add_impl :54-95) */
+      if(c->ras == 1)
+        for(int i = 0; i < ndp; ++i) ul[i] /= c->ras_w[idx[i]];
+      if(c->ras == 2)
+      {
+        const unsigned char *fl = c->ras_flags + (long)j * ndp;
+        for(int i = 0; i < ndp; ++i)
+          if(!fl[i]) solution[idx[i]] += ul[i];
+      }
+      else
+        for(int i = 0; i < ndp; ++i) solution[idx[i]] += ul[i];
     }
     free(idx);
     free(rl);
     free(ul);
   }
+}
+
+/*
+ * Restricted additive Schwarz data.
+ * compute_ras_weights (schwarz_preconditioner.templates.h:126-165): every patch
+ * scatter_adds a local vector of ones, so w[i] = number of patches containing i.
+ * compute_restricted_dofs_impl (dof_info.templates.h:192-313), DGQ layout: for
+ * each DoF the pairs (patch, local index i) containing it are ordered by
+ * (distance of i from the patch centre, i, patch) and only the first keeps the
+ * DoF; the distance is static_cast<unsigned>(max_d |k/2 - 0.5 - m_d|) for the
+ * patch multi-index m of i (comp_distance_impl_dgq, :233-244).
+ * Additive, scalar (one component) only, as in the reference (:207-215).
+ */
+int or_set_ras(or_ctx *c, int mode)
+{
+  if(mode < 0 || mode > 2) return 1;
+  if(mode && (c->variant != OR_ADDITIVE || c->ncomp != 1 || !c->ready)) return 1;
+  c->ras = mode;
+  free(c->ras_w);
+  free(c->ras_flags);
+  c->ras_w = NULL;
+  c->ras_flags = NULL;
+  if(!mode) return 0;
+  const int ndp = c->ndp, dim = c->dim, k = c->k;
+  uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * ndp);
+  c->ras_w = (double *)calloc(c->N, sizeof(double));
+  c->ras_flags = (unsigned char *)malloc((size_t)c->P * ndp);
+  memset(c->ras_flags, 1, (size_t)c->P * ndp);
+  /* best (distance, index, patch) per DoF, and its position */
+  unsigned *best = (unsigned *)malloc(sizeof(unsigned) * 3 * c->N);
+  long *pos = (long *)malloc(sizeof(long) * c->N);
+  for(long i = 0; i < c->N; ++i) { best[3 * i] = 0xffffffffu; pos[i] = -1; }
+  const double mid = k / 2.0 - 0.5;
+  for(int j = 0; j < c->P; ++j)
+  {
+    patch_dofs(c, c->vtx + j * dim, idx);
+    for(int i = 0; i < ndp; ++i)
+    {
+      const uint32_t g = idx[i];
+      c->ras_w[g] += 1.0;
+      double dist = 0.0;
+      int t = i;
+      for(int d = 0; d < dim; ++d)
+      {
+        const double diff = fabs(mid - (double)(t % k));
+        if(diff > dist) dist = diff;
+        t /= k;
+      }
+      const unsigned key[3] = {(unsigned)dist, (unsigned)i, (unsigned)j};
+      unsigned *b = best + 3 * (long)g;
+      int less = 0;
+      for(int q = 0; q < 3; ++q)
+        if(key[q] != b[q]) { less = key[q] < b[q]; break; }
+      if(less)
+      {
+        b[0] = key[0]; b[1] = key[1]; b[2] = key[2];
+        pos[g] = (long)j * ndp + i;
+      }
+    }
+  }
+  for(long g = 0; g < c->N; ++g)
+    if(pos[g] >= 0) c->ras_flags[pos[g]] = 0;
+  free(best);
+  free(pos);
+  free(idx);
+  return 0;
+}
+
+void or_ras_weights(const or_ctx *c, double *out)
+{
+  for(long i = 0; i < c->N; ++i) out[i] = c->ras_w ? c->ras_w[i] : 0.0;
+}
+
+void or_restricted_flags(const or_ctx *c, unsigned char *out)
+{
+  if(c->ras_flags) memcpy(out, c->ras_flags, (size_t)c->P * c->ndp);
+  else memset(out, 0, (size_t)c->P * c->ndp);
 }
 
 /* vmult, schwarz_preconditioner.templates.h:209-239 with

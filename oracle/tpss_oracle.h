@@ -61,6 +61,10 @@ int or_n_components(const or_ctx *c);
  * matrix, `sweeps` sweeps); call before or_setup */
 void or_set_local_solver(or_ctx *c, int kind, int sweeps);
 void or_fit_errors(const or_ctx *c, double *out);
+/* restricted additive Schwarz: 0 off, 1 weights, 2 boolean (after or_setup) */
+int or_set_ras(or_ctx *c, int mode);
+void or_ras_weights(const or_ctx *c, double *out);
+void or_restricted_flags(const or_ctx *c, unsigned char *out);
 int or_fit_fallbacks(const or_ctx *c);
 void or_patch_local_matrix(const or_ctx *c, int patch, double *A);
 double or_separable_fit(int dim, int k, const double *A, double *M, double *L, int sweeps);

@@ -24,6 +24,7 @@ COLOR_NONE, COLOR_TILED, COLOR_REDBLACK = 0, 1, 2
 F64, F32, BF16 = 0, 1, 2
 POISSON, ELASTICITY = 0, 1
 LOCAL_SURROGATE, LOCAL_FIT = 0, 1
+RAS_NONE, RAS_WEIGHTS, RAS_BOOLEAN = 0, 1, 2
 
 #: TPSS::lookup_damping_factor (TPSS.h:120-163) for vertex patches
 DEFAULT_DAMPING = {(ADDITIVE, 2): 0.25, (ADDITIVE, 3): 0.15,
@@ -48,7 +49,8 @@ class SmootherDesc(ctypes.Structure):
                 ("relaxation", ctypes.c_double), ("reverse", ctypes.c_int),
                 ("symmetrized", ctypes.c_int), ("damping_factor", ctypes.c_double),
                 ("n_steps", ctypes.c_int), ("local_precision", ctypes.c_int),
-                ("local_solver", ctypes.c_int), ("fit_sweeps", ctypes.c_int)]
+                ("local_solver", ctypes.c_int), ("fit_sweeps", ctypes.c_int),
+                ("ras", ctypes.c_int)]
 
 
 class TimeInfo(ctypes.Structure):
@@ -79,6 +81,7 @@ EXPORTS = [
     "tpss_precond_Tvmult_add", "tpss_smoother_step", "tpss_smoother_vmult",
     "tpss_smoother_set_patch_damping", "tpss_smoother_info", "tpss_smoother_color_ptr",
     "tpss_smoother_patch_vertices", "tpss_smoother_patch_dof_indices", "tpss_smoother_patch_h",
+    "tpss_smoother_restricted_dof_flags",
     "tpss_smoother_fit_errors",
     "tpss_smoother_get_timings", "tpss_smoother_set_timing", "tpss_smoother_set_check_finite",
     "tpss_smoother_finite_status",
@@ -91,7 +94,8 @@ EXPORTS = [
     "tpss_mlp_create", "tpss_mlp_destroy", "tpss_mlp_n_params", "tpss_mlp_set_params",
     "tpss_mlp_get_params", "tpss_mlp_forward", "tpss_mlp_loss_grad", "tpss_mlp_train",
     "tpss_smoother_patch_features", "tpss_smoother_optimal_damping", "tpss_smoother_set_damping_mlp",
-    "tpss_ksvd_batched",
+    "tpss_ksvd_batched", "tpss_ksvd_sum_batched", "tpss_kron_vmult", "tpss_kron_table",
+    "tpss_kron_pinv", "tpss_kcp_batched",
 ]
 
 _VP = ctypes.c_void_p
@@ -127,6 +131,7 @@ def _load():
         "tpss_smoother_patch_vertices": (ctypes.c_int, [_VP, ip]),
         "tpss_smoother_patch_dof_indices": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint32)]),
         "tpss_smoother_patch_h": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
+        "tpss_smoother_restricted_dof_flags": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint8)]),
         "tpss_smoother_get_timings": (ctypes.c_int, [_VP, ctypes.POINTER(TimeInfo), ip]),
         "tpss_smoother_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
         "tpss_smoother_set_check_finite": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -169,6 +174,15 @@ def _load():
         "tpss_ksvd_batched": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP,
                                              _VP, _VP]),
+        "tpss_ksvd_sum_batched": (ctypes.c_int, [_VP, _VP] + [ctypes.c_int] * 7 + [_VP] * 4),
+        "tpss_kron_vmult": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ip, ip, ctypes.c_int,
+                                           _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, _VP]),
+        "tpss_kron_table": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ip, ip, ctypes.c_int,
+                                           _VP, ctypes.c_int, _VP, _VP]),
+        "tpss_kron_pinv": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_int, _VP, _VP]),
+        "tpss_kcp_batched": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ip, ip, ctypes.c_int,
+                                            ctypes.c_int, _VP, _VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -275,10 +289,10 @@ def _vertices_ptr(vertices):
 
 
 def _smoother_desc(variant, coloring, relaxation, reverse, symmetrized, damping_factor, n_steps,
-                   local_precision, local_solver=LOCAL_SURROGATE, fit_sweeps=0):
+                   local_precision, local_solver=LOCAL_SURROGATE, fit_sweeps=0, ras=RAS_NONE):
     return SmootherDesc(variant, coloring, float(relaxation), int(reverse), int(symmetrized),
                         float(damping_factor), int(n_steps), int(local_precision),
-                        int(local_solver), int(fit_sweeps))
+                        int(local_solver), int(fit_sweeps), int(ras))
 
 
 class Level:
@@ -332,7 +346,8 @@ class SchwarzPreconditioner:
 
     def __init__(self, level, variant=ADDITIVE, coloring=None, relaxation=None, reverse=False,
                  symmetrized=False, damping_factor=1.0, n_steps=1, local_precision=None,
-                 local_solver=LOCAL_SURROGATE, fit_sweeps=0, _handle=None, _owner=None):
+                 local_solver=LOCAL_SURROGATE, fit_sweeps=0, ras=RAS_NONE, _handle=None,
+                 _owner=None):
         if coloring is None:
             coloring = COLOR_NONE if variant == ADDITIVE else COLOR_REDBLACK
         if relaxation is None:
@@ -342,7 +357,7 @@ class SchwarzPreconditioner:
         self.level = level
         self.desc = _smoother_desc(variant, coloring, relaxation, reverse, symmetrized,
                                    damping_factor, n_steps, local_precision, local_solver,
-                                   fit_sweeps)
+                                   fit_sweeps, ras)
         if _handle is not None:
             # borrowed from _owner (a GMGSolver), which must outlive this wrapper
             self._owner = _owner
@@ -452,6 +467,14 @@ class SchwarzPreconditioner:
             self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
         return out
 
+    def restricted_dof_flags(self):
+        """(n_patches, n_dofs_per_patch) uint8: 1 where the restricted prolongation skips"""
+        import numpy as np
+        out = np.zeros((self.n_patches, self.n_dofs_per_patch), dtype=np.uint8)
+        _check(lib.tpss_smoother_restricted_dof_flags(
+            self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
     def patch_h(self):
         import numpy as np
         out = np.zeros((self.n_patches, self.level.dim, 2))
@@ -492,7 +515,7 @@ class GMGSolver:
     def __init__(self, dim, degree, n_cells_coarse, n_levels, ip_factor=1.0, vertices=None,
                  mg_precision=F64, variant=MULTIPLICATIVE, coloring=None, relaxation=None,
                  damping_factor=1.0, n_steps=1, local_precision=None, coarse_tol=1e-12,
-                 elasticity=None, local_solver=LOCAL_SURROGATE, fit_sweeps=0):
+                 elasticity=None, local_solver=LOCAL_SURROGATE, fit_sweeps=0, ras=RAS_NONE):
         if coloring is None:
             coloring = COLOR_NONE if variant == ADDITIVE else COLOR_REDBLACK
         if relaxation is None:
@@ -501,7 +524,7 @@ class GMGSolver:
             local_precision = mg_precision
         self._varr, vp = _vertices_ptr(vertices)
         sd = _smoother_desc(variant, coloring, relaxation, False, False, damping_factor, n_steps,
-                            local_precision, local_solver, fit_sweeps)
+                            local_precision, local_solver, fit_sweeps, ras)
         eq, mu, lam = (POISSON, 0.0, 0.0) if elasticity is None else (ELASTICITY,) + tuple(
             float(v) for v in elasticity)
         self.desc = GMGDesc(dim, degree, n_cells_coarse, n_levels, float(ip_factor), vp,
@@ -645,6 +668,118 @@ def ksvd(A, m1, n1, m0, n0, rank):
     B0 = dB0.numpy().reshape(batch, rank, m0, n0)
     s = ds.numpy().reshape(batch, rank)
     return (B1[0], B0[0], s[0]) if single else (B1, B0, s)
+
+
+def _kron_pack(batch_terms):
+    """(order, rank, m, n, packed factors) of a batch of Kronecker sums given as
+    lists over rank of [A_0, ..., A_{order-1}] (A_0 rightmost)"""
+    import numpy as np
+    first = batch_terms[0]
+    order, rank = len(first[0]), len(first)
+    m = (ctypes.c_int * 3)(*[np.shape(A)[0] for A in first[0]])
+    n = (ctypes.c_int * 3)(*[np.shape(A)[1] for A in first[0]])
+    packed = np.concatenate([np.asarray(A, dtype=np.float64).ravel()
+                             for terms in batch_terms for t in terms for A in t])
+    return order, rank, m, n, packed
+
+
+def kron_vmult_batched(batch_terms, X, transpose=False, add_to=None):
+    """Y[b] = A_b X[b] (or A_b^T X[b]; or add_to[b] + ...) for the Kronecker sums A_b
+    (TensorProductMatrix basic state vmult / Tvmult / vmult_add) on the device."""
+    import numpy as np
+    order, rank, m, n, packed = _kron_pack(batch_terms)
+    batch = len(batch_terms)
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(batch, -1)
+    nout = int(np.prod([n[d] if transpose else m[d] for d in range(order)]))
+    dF, dX = DeviceVector.from_numpy(packed), DeviceVector.from_numpy(X.ravel())
+    if add_to is None:
+        dY, add = DeviceVector(batch * nout), 0
+    else:
+        dY, add = DeviceVector.from_numpy(np.asarray(add_to, dtype=np.float64).ravel()), 1
+    _check(lib.tpss_kron_vmult(order, rank, m, n, batch, _ptr(dF), _ptr(dX), _ptr(dY),
+                               int(transpose), add, None))
+    return dY.numpy().reshape(batch, nout)
+
+
+def kron_vmult(terms, x, transpose=False, add_to=None):
+    return kron_vmult_batched([terms], [x], transpose,
+                              None if add_to is None else [add_to])[0]
+
+
+def kron_table(terms_or_batch, transpose=False):
+    """dense matrices (as_table / as_transpose_table) of a batch of Kronecker sums;
+    a single sum (list of rank-1 terms) gives a batch of one"""
+    import numpy as np
+    batch_terms = terms_or_batch if np.ndim(terms_or_batch[0][0][0]) == 2 else [terms_or_batch]
+    order, rank, m, n, packed = _kron_pack(batch_terms)
+    M = int(np.prod([m[d] for d in range(order)]))
+    N = int(np.prod([n[d] for d in range(order)]))
+    dF, dT = DeviceVector.from_numpy(packed), DeviceVector(len(batch_terms) * M * N)
+    _check(lib.tpss_kron_table(order, rank, m, n, len(batch_terms), _ptr(dF), int(transpose),
+                               _ptr(dT), None))
+    shape = (N, M) if transpose else (M, N)
+    return dT.numpy().reshape((len(batch_terms),) + shape)
+
+
+def pinv_batched(A, threshold=0.0, kernel_size=-1):
+    """SVD pseudo-inverses of a batch of square matrices (InverseTable)"""
+    import numpy as np
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    batch, n = A.shape[0], A.shape[1]
+    dA, dP = DeviceVector.from_numpy(A.ravel()), DeviceVector(A.size)
+    _check(lib.tpss_kron_pinv(_ptr(dA), n, batch, float(threshold), int(kernel_size), _ptr(dP),
+                              None))
+    return dP.numpy().reshape(A.shape)
+
+
+def ksvd_sum(big, small, rank):
+    """Kronecker SVD of sum_i big_i (x) small_i (batch, in_rank, m, n) arrays, or of
+    one sum given as lists -> (B1, B0, sigma) like ksvd()"""
+    import numpy as np
+    big = np.ascontiguousarray(big, dtype=np.float64)
+    small = np.ascontiguousarray(small, dtype=np.float64)
+    single = big.ndim == 3
+    if single:
+        big, small = big[None], small[None]
+    batch, in_rank, m1, n1 = big.shape
+    m0, n0 = small.shape[2:]
+    dA, dB = DeviceVector.from_numpy(big.ravel()), DeviceVector.from_numpy(small.ravel())
+    d1, d0, ds = (DeviceVector(batch * rank * m1 * n1), DeviceVector(batch * rank * m0 * n0),
+                  DeviceVector(batch * rank))
+    _check(lib.tpss_ksvd_sum_batched(_ptr(dA), _ptr(dB), batch, in_rank, m1, n1, m0, n0, rank,
+                                     _ptr(d1), _ptr(d0), _ptr(ds), None))
+    B1 = d1.numpy().reshape(batch, rank, m1, n1)
+    B0 = d0.numpy().reshape(batch, rank, m0, n0)
+    s = ds.numpy().reshape(batch, rank)
+    return (B1[0], B0[0], s[0]) if single else (B1, B0, s)
+
+
+def kcp(batch_terms, rank, iterations=0):
+    """CP-ALS (compute_kcp) of a batch of sums of 3-fold Kronecker products, each a
+    list over input terms of [C_i, B_i, A_i]; returns rank-R lists in that order"""
+    import numpy as np
+    shapes = [np.shape(A) for A in batch_terms[0][0]]
+    in_rank = len(batch_terms[0])
+    packed = np.concatenate([np.asarray(A, dtype=np.float64).ravel()
+                             for terms in batch_terms for t in terms for A in t])
+    m = (ctypes.c_int * 3)(*[s_[0] for s_ in shapes])
+    n = (ctypes.c_int * 3)(*[s_[1] for s_ in shapes])
+    term = sum(a * b for a, b in shapes)
+    dIn, dOut = DeviceVector.from_numpy(packed), DeviceVector(len(batch_terms) * rank * term)
+    _check(lib.tpss_kcp_batched(_ptr(dIn), len(batch_terms), in_rank, m, n, rank,
+                                int(iterations), _ptr(dOut), None))
+    out = dOut.numpy().reshape(len(batch_terms), rank, term)
+    res = []
+    for b in range(len(batch_terms)):
+        terms = []
+        for r in range(rank):
+            off, facs = 0, []
+            for s_ in shapes:
+                facs.append(out[b, r, off:off + s_[0] * s_[1]].reshape(s_))
+                off += s_[0] * s_[1]
+            terms.append(facs)
+        res.append(terms)
+    return res
 
 
 def version():

@@ -841,9 +841,51 @@ k_patch_fd(const VT *__restrict__ src, VT *__restrict__ dst, CT *__restrict__ ub
   }
 }
 
+// Restricted additive Schwarz (rscatter_add, patch_transfer.templates.h:140-158):
+// of the patches that contain DoF (cell c, local node l), the one that keeps it
+// in the restricted prolongation -- among the patches at interior vertices of
+// the cell, the one whose patch multi-index m of the DoF minimises
+// (max_d floor|K/2 - 1/2 - m_d|, lexicographic index of m), the order of
+// compute_restricted_dofs_impl (dof_info.templates.h:192-313; ties cannot remain,
+// two patches never hold one DoF at the same multi-index).  Returns the DoF's
+// slot in that patch: bit d set if the cell is the upper one in direction d.
+template<int DIM, int NP>
+__device__ __forceinline__ int ras_owner_slot(const int *c, const int *l, const int n)
+{
+  int best = -1;
+  int bd = 1 << 30, bi = 1 << 30;
+#pragma unroll
+  for(int slot = 0; slot < (1 << DIM); ++slot)
+  {
+    bool ok = true;
+    int dist = 0, flat = 0, st = 1;
+#pragma unroll
+    for(int d = 0; d < DIM; ++d)
+    {
+      const int bit = (slot >> d) & 1;
+      const int vd = c[d] + 1 - bit;
+      ok = ok && vd >= 1 && vd <= n - 1;
+      // |NP - 1/2 - m_d| rounded down: NP - 1 - l_d (lower cell), l_d (upper cell)
+      dist = max(dist, bit ? l[d] : NP - 1 - l[d]);
+      flat += (bit * NP + l[d]) * st;
+      st *= 2 * NP;
+    }
+    if(ok && (dist < bd || (dist == bd && flat < bi)))
+    {
+      best = slot;
+      bd = dist;
+      bi = flat;
+    }
+  }
+  return best;
+}
+
 // dst[i] (=|+=) alpha * sum over the patches containing DoF i of their buffered
-// contribution (additive Schwarz without atomics).
-template<int DIM, int NP, typename VT, typename CT, bool ACCUMULATE>
+// contribution (additive Schwarz without atomics).  RAS = 1: the sum divided by
+// the number of those patches (use_ras_weights, schwarz_preconditioner.templates.h:
+// 636-641); RAS = 2: only the contribution of the patch that keeps the DoF
+// (use_ras_boolean_weights, rscatter_add, :646-649).
+template<int DIM, int NP, typename VT, typename CT, bool ACCUMULATE, int RAS = 0>
 __global__ void __launch_bounds__(256)
 k_avp_sum(const CT *__restrict__ ubuf, VT *__restrict__ dst, const int n, const int64_t N,
           const double alpha)
@@ -855,7 +897,7 @@ k_avp_sum(const CT *__restrict__ ubuf, VT *__restrict__ dst, const int n, const 
   const int l = int(i % NDC);
   int c[3] = {0, 0, 0};
   {
-    int t = int(cell);  // cells < n^3 <= 1000^3 < 2^31
+    int t = int(cell);  // cells < 2^24
     for(int d = 0; d < DIM; ++d)
     {
       c[d] = t % n;
@@ -863,22 +905,62 @@ k_avp_sum(const CT *__restrict__ ubuf, VT *__restrict__ dst, const int n, const 
     }
   }
   CT s = 0;
-#pragma unroll
-  for(int slot = 0; slot < (1 << DIM); ++slot)
+  if constexpr(RAS == 2)
   {
-    bool ok = true;
+    int lm[3] = {0, 0, 0};
+    for(int d = 0, t = l; d < DIM; ++d, t /= NP) lm[d] = t % NP;
+    const int slot = ras_owner_slot<DIM, NP>(c, lm, n);
+    s = ubuf[(cell * (1 << DIM) + slot) * NDC + l];
+  }
+  else
+  {
+    int count = 0;
 #pragma unroll
-    for(int d = 0; d < DIM; ++d)
+    for(int slot = 0; slot < (1 << DIM); ++slot)
     {
-      const int vd = c[d] + 1 - ((slot >> d) & 1);
-      ok = ok && vd >= 1 && vd <= n - 1;
+      bool ok = true;
+#pragma unroll
+      for(int d = 0; d < DIM; ++d)
+      {
+        const int vd = c[d] + 1 - ((slot >> d) & 1);
+        ok = ok && vd >= 1 && vd <= n - 1;
+      }
+      if(ok)
+      {
+        s += ubuf[(cell * (1 << DIM) + slot) * NDC + l];
+        ++count;
+      }
     }
-    if(ok) s += ubuf[(cell * (1 << DIM) + slot) * NDC + l];
+    if constexpr(RAS == 1) s /= CT(count);
   }
   if(ACCUMULATE)
     dst[i] = VT(double(dst[i]) + alpha * double(s));
   else
     dst[i] = VT(alpha * double(s));
+}
+
+// restricted-DoF flags of every patch (DoFInfo::restricted_dof_flags_patchwise,
+// dof_info.templates.h:192-313): out[j * K^DIM + i] = 1 if patch j does not keep
+// its patch DoF i in the restricted prolongation
+template<int DIM, int NP>
+__global__ void k_restricted_flags(const uint32_t *__restrict__ vtx, const int P, const int n,
+                                   uint8_t *__restrict__ out)
+{
+  constexpr int K = 2 * NP, NPD = cpow(K, DIM);
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(g >= int64_t(P) * NPD) return;
+  const int j = int(g / NPD), i = int(g % NPD);
+  int v[3] = {1, 1, 1};
+  unpack_vertex(vtx[j], v);
+  int c[3] = {0, 0, 0}, lm[3] = {0, 0, 0}, slot = 0;
+  for(int d = 0, t = i; d < DIM; ++d, t /= K)
+  {
+    const int m = t % K, bit = m / NP;
+    lm[d] = m % NP;
+    c[d] = v[d] - 1 + bit;
+    slot |= bit << d;
+  }
+  out[g] = uint8_t(ras_owner_slot<DIM, NP>(c, lm, n) != slot);
 }
 
 // ----------------------------------------------------------------------------

@@ -3,6 +3,10 @@
 //   A ~ sum_i B1_i (x) B0_i,  A of size (m1 m0) x (n1 n0),
 // from the leading singular triplets of the rearrangement
 //   R(A)[i1*n1 + j1, i0*n0 + j0] = A[i1*m0 + i0, j1*n0 + j0]   (reshuffle_full, :92-117).
+// The input is either A itself or a sum of Kronecker products
+//   A = sum_{i < in_rank} Abig_i (x) Bsmall_i,  R(A) = sum_i vec(Abig_i) vec(Bsmall_i)^T
+// (compute_ksvd for std::vector<std::array<Table, 2>>, :306-402, whose Lanczos
+// products rankk_vector_multiplication never form A).
 // Golub-Kahan-Lanczos bidiagonalisation of R(A) started from e_0 with full
 // reorthogonalisation, r^2 + 10 steps (or until breakdown), then the SVD of
 // the small bidiagonal matrix (one-sided Jacobi) and the factors scaled by
@@ -46,17 +50,25 @@ __device__ __forceinline__ double rearranged(const double *A, int m1, int n1, in
   return A[size_t(i1 * m0 + i0) * (n1 * n0) + j1 * n0 + j0];
 }
 
+constexpr int KS_MAXIN = 16;
+
+// SUM: Aall = the big factors (batch x in_rank x Rn), Ball = the small ones
+// (batch x in_rank x Cn); otherwise Aall = the dense matrices
+template<bool SUM>
 __global__ void __launch_bounds__(KS_T)
-k_ksvd(const double *__restrict__ Aall, const int m1, const int n1, const int m0, const int n0,
-       const int rank, const int iters, double *__restrict__ work, double *__restrict__ B1,
-       double *__restrict__ B0, double *__restrict__ sigma)
+k_ksvd(const double *__restrict__ Aall, const double *__restrict__ Ball, const int in_rank,
+       const int m1, const int n1, const int m0, const int n0, const int rank, const int iters,
+       double *__restrict__ work, double *__restrict__ B1, double *__restrict__ B0,
+       double *__restrict__ sigma)
 {
   __shared__ double sh[KS_T];
   __shared__ double alpha[KS_MAXIT + 1], beta[KS_MAXIT + 2];
+  __shared__ double dots[KS_MAXIN];
   __shared__ int steps;
   const int Rn = m1 * n1, Cn = m0 * n0;
   const size_t b = blockIdx.x;
-  const double *A = Aall + b * size_t(Rn) * Cn;
+  const double *A = SUM ? Aall + b * size_t(in_rank) * Rn : Aall + b * size_t(Rn) * Cn;
+  const double *Bsm = SUM ? Ball + b * size_t(in_rank) * Cn : nullptr;
   // workspace per matrix: U[(iters+1) x Rn] (u_0 = 0), V[iters x Cn], P[Cn], Q[Rn]
   double *U = work + b * (size_t(iters + 1) * Rn + size_t(iters) * Cn + Cn + Rn);
   double *V = U + size_t(iters + 1) * Rn;
@@ -84,12 +96,30 @@ k_ksvd(const double *__restrict__ Aall, const int m1, const int n1, const int m0
     for(int i = threadIdx.x; i < Cn; i += KS_T) v[i] = bn > 0.0 ? P[i] / bn : 0.0;
     __syncthreads();
     // r = R v - beta u_it, orthogonalised against the previous u's
-    for(int row = threadIdx.x; row < Rn; row += KS_T)
+    if constexpr(SUM)
     {
-      double acc = 0.0;
-      for(int col = 0; col < Cn; ++col) acc += rearranged(A, m1, n1, m0, n0, row, col) * v[col];
-      Q[row] = acc - bn * U[size_t(it) * Rn + row];
+      for(int i = 0; i < in_rank; ++i)
+      {
+        double d = 0.0;
+        for(int col = threadIdx.x; col < Cn; col += KS_T) d += Bsm[size_t(i) * Cn + col] * v[col];
+        d = block_sum(d, sh);
+        if(threadIdx.x == 0) dots[i] = d;
+      }
+      __syncthreads();
+      for(int row = threadIdx.x; row < Rn; row += KS_T)
+      {
+        double acc = 0.0;
+        for(int i = 0; i < in_rank; ++i) acc += A[size_t(i) * Rn + row] * dots[i];
+        Q[row] = acc - bn * U[size_t(it) * Rn + row];
+      }
     }
+    else
+      for(int row = threadIdx.x; row < Rn; row += KS_T)
+      {
+        double acc = 0.0;
+        for(int col = 0; col < Cn; ++col) acc += rearranged(A, m1, n1, m0, n0, row, col) * v[col];
+        Q[row] = acc - bn * U[size_t(it) * Rn + row];
+      }
     __syncthreads();
     for(int j = 1; j <= it; ++j)
     {
@@ -108,12 +138,30 @@ k_ksvd(const double *__restrict__ Aall, const int m1, const int n1, const int m0
     for(int i = threadIdx.x; i < Rn; i += KS_T) un[i] = an > 0.0 ? Q[i] / an : 0.0;
     __syncthreads();
     // p = R^T u - alpha v, orthogonalised against the v's
-    for(int col = threadIdx.x; col < Cn; col += KS_T)
+    if constexpr(SUM)
     {
-      double acc = 0.0;
-      for(int row = 0; row < Rn; ++row) acc += rearranged(A, m1, n1, m0, n0, row, col) * un[row];
-      P[col] = acc - an * v[col];
+      for(int i = 0; i < in_rank; ++i)
+      {
+        double d = 0.0;
+        for(int row = threadIdx.x; row < Rn; row += KS_T) d += A[size_t(i) * Rn + row] * un[row];
+        d = block_sum(d, sh);
+        if(threadIdx.x == 0) dots[i] = d;
+      }
+      __syncthreads();
+      for(int col = threadIdx.x; col < Cn; col += KS_T)
+      {
+        double acc = 0.0;
+        for(int i = 0; i < in_rank; ++i) acc += Bsm[size_t(i) * Cn + col] * dots[i];
+        P[col] = acc - an * v[col];
+      }
     }
+    else
+      for(int col = threadIdx.x; col < Cn; col += KS_T)
+      {
+        double acc = 0.0;
+        for(int row = 0; row < Rn; ++row) acc += rearranged(A, m1, n1, m0, n0, row, col) * un[row];
+        P[col] = acc - an * v[col];
+      }
     __syncthreads();
     for(int j = 0; j <= it; ++j)
     {
@@ -267,8 +315,31 @@ int tpss_ksvd_batched(const double *A, int batch, int m1, int n1, int m0, int n0
     const size_t per = (iters + 1) * Rn + iters * Cn + Cn + Rn;
     DeviceArray<double> work{per * size_t(batch)};
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_ksvd, dim3(unsigned(batch)), dim3(KS_T), 0, s, A, m1, n1, m0, n0, rank,
-                       iters, work.ptr, B1, B0, sigma);
+    hipLaunchKernelGGL(k_ksvd<false>, dim3(unsigned(batch)), dim3(KS_T), 0, s, A, nullptr, 0, m1,
+                       n1, m0, n0, rank, iters, work.ptr, B1, B0, sigma);
+    TPSS_HIP(hipGetLastError());
+    TPSS_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int tpss_ksvd_sum_batched(const double *Abig, const double *Bsmall, int batch, int in_rank, int m1,
+                          int n1, int m0, int n0, int rank, double *B1, double *B0, double *sigma,
+                          void *stream)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(Abig && Bsmall && B1 && B0 && sigma, "null argument");
+    TPSS_REQUIRE(batch >= 0 && m1 > 0 && n1 > 0 && m0 > 0 && n0 > 0 && rank >= 1,
+                 "invalid dimensions");
+    TPSS_REQUIRE(in_rank >= 1 && in_rank <= KS_MAXIN, "in_rank must be in 1..16");
+    const int iters = std::min(KS_MAXIT, std::min(rank * rank + 10, std::min(m1 * n1, m0 * n0)));
+    TPSS_REQUIRE(rank <= iters, "rank exceeds the Lanczos space");
+    if(batch == 0) return;
+    const size_t Rn = size_t(m1) * n1, Cn = size_t(m0) * n0;
+    const size_t per = (iters + 1) * Rn + iters * Cn + Cn + Rn;
+    DeviceArray<double> work{per * size_t(batch)};
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_ksvd<true>, dim3(unsigned(batch)), dim3(KS_T), 0, s, Abig, Bsmall,
+                       in_rank, m1, n1, m0, n0, rank, iters, work.ptr, B1, B0, sigma);
     TPSS_HIP(hipGetLastError());
     TPSS_HIP(hipStreamSynchronize(s));
   });

@@ -514,12 +514,24 @@ void avp_sum(tpss_smoother *sm, const CT *ubuf, VT *dst, bool accumulate, double
   dispatch_dim_np(lv->dim, lv->np, [&](auto D, auto NPc) {
     constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
     const unsigned g = blocks_for(lv->Ns, 256);
+    auto launch = [&](auto ACC, auto RASc) {
+      hipLaunchKernelGGL((k_avp_sum<DIM, NP, VT, CT, decltype(ACC)::value != 0,
+                                    decltype(RASc)::value>),
+                         dim3(g), dim3(256), 0, s, ubuf, dst, lv->n, lv->Ns, alpha);
+    };
+    const int ras = sm->desc.ras;
     if(accumulate)
-      hipLaunchKernelGGL((k_avp_sum<DIM, NP, VT, CT, true>), dim3(g), dim3(256), 0, s, ubuf, dst,
-                         lv->n, lv->Ns, alpha);
+    {
+      if(ras == 1) launch(ic<1>{}, ic<1>{});
+      else if(ras == 2) launch(ic<1>{}, ic<2>{});
+      else launch(ic<1>{}, ic<0>{});
+    }
     else
-      hipLaunchKernelGGL((k_avp_sum<DIM, NP, VT, CT, false>), dim3(g), dim3(256), 0, s, ubuf,
-                         dst, lv->n, lv->Ns, alpha);
+    {
+      if(ras == 1) launch(ic<0>{}, ic<1>{});
+      else if(ras == 2) launch(ic<0>{}, ic<2>{});
+      else launch(ic<0>{}, ic<0>{});
+    }
     TPSS_HIP(hipGetLastError());
   });
 }
@@ -579,8 +591,12 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
   SectionTimer step_timer(sm, 4, s);
   if(sm->desc.variant == TPSS_ADDITIVE)
   {
-    sm->ubuf.resize(size_t(lv->Ns) * (1 << lv->dim) * sizeof(CT));
-    CT *ub = (CT *)sm->ubuf.ptr;
+    CT *ub = (CT *)sm->scratch;
+    if(!ub)
+    {
+      sm->ubuf.resize(size_t(lv->Ns) * (1 << lv->dim) * sizeof(CT));
+      ub = (CT *)sm->ubuf.ptr;
+    }
     {
       SectionTimer t(sm, 1, s);
       for(int c = 0; c < ls.ncomp; ++c)
@@ -611,8 +627,12 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
     if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
     return;
   }
-  sm->res.resize(size_t(lv->N) * sizeof(VT));
-  VT *res = (VT *)sm->res.ptr;
+  VT *res = (VT *)sm->scratch;
+  if(!res)
+  {
+    sm->res.resize(size_t(lv->N) * sizeof(VT));
+    res = (VT *)sm->res.ptr;
+  }
   fill_zero(out, lv->vec_prec, lv->N, s);
   for(size_t k = 0; k < seq.size(); ++k)
   {
@@ -759,6 +779,21 @@ void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_
   }
 }
 
+// device scratch (bytes) one precond vmult (dst overwritten) needs: the additive
+// slot buffer, or the residual of the multiplicative sweeps that update it
+size_t precond_scratch_bytes(const tpss_smoother *sm)
+{
+  const tpss_level *lv = sm->level;
+  const LocalSolvers &ls = *sm->ls;
+  const int lp = sm->desc.local_precision;
+  const size_t ct = lp == TPSS_F64 ? 8 : 4;
+  if(sm->desc.variant == TPSS_ADDITIVE) return size_t(lv->Ns) * (size_t(1) << lv->dim) * ct;
+  if(lp == TPSS_BF16) return 0;
+  if(!lv->distorted && lv->ncomp == 1 && ls.coloring == TPSS_COLOR_REDBLACK && sm->fused_mvp)
+    return 0;
+  return size_t(lv->N) * elem_size(lv->vec_prec);
+}
+
 // throws (once) if a check kernel of an earlier call has flagged its output
 void report_nonfinite(tpss_smoother *sm)
 {
@@ -821,6 +856,10 @@ void validate_desc(const tpss_smoother_desc *d)
   TPSS_REQUIRE(d->fit_sweeps >= 0 && d->fit_sweeps <= 100, "fit_sweeps must be in 0..100");
   TPSS_REQUIRE(!(d->variant == TPSS_MULTIPLICATIVE && d->coloring == TPSS_COLOR_NONE),
                "the multiplicative variant needs a coloring (RedBlack for DG)");
+  TPSS_REQUIRE(d->ras >= 0 && d->ras <= 2, "ras must be 0, 1 or 2");
+  // dof_info.templates.h:213-214: "Additive scheme is only supported."
+  TPSS_REQUIRE(d->ras == 0 || d->variant == TPSS_ADDITIVE,
+               "restricted additive Schwarz needs the additive variant");
 }
 
 }  // namespace tpss
@@ -840,12 +879,16 @@ int tpss_smoother_create(tpss_level *level, const tpss_smoother_desc *desc, tpss
       throw Error(TPSS_E_UNSUPPORTED, "linear elasticity: local precision F64 or F32");
     if(level->ncomp > 1 && desc->local_solver == TPSS_LOCAL_FIT)
       throw Error(TPSS_E_UNSUPPORTED, "fitted local solvers: Poisson levels only");
+    // dof_info.templates.h:208: "TODO more than scalar elements"
+    if(level->ncomp > 1 && desc->ras != 0)
+      throw Error(TPSS_E_UNSUPPORTED, "restricted additive Schwarz: scalar (Poisson) levels only");
     auto sm = std::make_unique<tpss_smoother>();
     sm->level = level;
     sm->desc = *desc;
     sm->ls = build_local_solvers(level, *desc, nullptr);
     sm->seq = color_sequence(sm->ls->n_colors, desc->reverse != 0, desc->symmetrized != 0);
     sm->seq_t.assign(sm->seq.rbegin(), sm->seq.rend());
+    sm->nonfinite.ensure();
     sm->timers[2].time = sm->ls->setup_seconds;
     sm->timers[2].n_calls = 1;
     *out = sm.release();
@@ -867,6 +910,7 @@ int tpss_smoother_clone_shallow(const tpss_smoother *other, const tpss_smoother_
     sm->seq = color_sequence(sm->ls->n_colors, desc->reverse != 0, desc->symmetrized != 0);
     sm->seq_t.assign(sm->seq.rbegin(), sm->seq.rend());
     sm->check_finite = other->check_finite;
+    sm->nonfinite.ensure();
     if(other->use_omega)
     {
       sm->d_omega.resize(other->d_omega.n);
@@ -1003,6 +1047,24 @@ int tpss_smoother_patch_dof_indices(const tpss_smoother *sm, uint32_t *out)
   return guarded([&] {
     TPSS_REQUIRE(sm && out, "null argument");
     patch_dof_indices_device(*sm->ls, out);
+  });
+}
+
+int tpss_smoother_restricted_dof_flags(const tpss_smoother *sm, uint8_t *out)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && out, "null argument");
+    const LocalSolvers &ls = *sm->ls;
+    TPSS_REQUIRE(ls.ncomp == 1, "restricted DoF flags: scalar levels only");
+    const int64_t total = int64_t(ls.P) * ipow(ls.K, ls.dim);
+    DeviceArray<uint8_t> d_out{size_t(total)};
+    dispatch_dim_np(ls.dim, ls.np, [&](auto D, auto NPc) {
+      constexpr int DIM = decltype(D)::value, NP = decltype(NPc)::value;
+      hipLaunchKernelGGL((k_restricted_flags<DIM, NP>), dim3(blocks_for(total, 256)), dim3(256), 0,
+                         nullptr, ls.d_vtx.ptr, ls.P, ls.n, d_out.ptr);
+      TPSS_HIP(hipGetLastError());
+    });
+    TPSS_HIP(hipMemcpy(out, d_out.ptr, size_t(total), hipMemcpyDeviceToHost));
   });
 }
 

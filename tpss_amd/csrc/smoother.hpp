@@ -92,12 +92,18 @@ struct tpss_smoother
   std::vector<PendingSection> pending;
   // scratch (bytes)
   tpss::DeviceArray<char> res, tmp2, rich_r, ubuf;
+  // caller-provided device scratch for the current call, used instead of res /
+  // ubuf when set (the allocation-free entry point)
+  char *scratch = nullptr;
 };
 
 namespace tpss
 {
 void precond_vmult_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
                        bool accumulate, double alpha, bool transpose = false);
+size_t precond_scratch_bytes(const tpss_smoother *sm);
+void report_nonfinite(tpss_smoother *sm);
+void check_finite(tpss_smoother *sm, const void *x, hipStream_t s);
 void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
                        bool x_is_zero = false);
 template<typename VT>
