@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run 5 FP64 -> FP32 conversions of N values (8N bytes read, 4N "
                          "written): the reference kernel for scaling HBM counter readings")
+    ap.add_argument("--write-traffic", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
+                    help="only turn four rocprofv3 --pmc counter CSVs (FETCH_SIZE, WRITE_SIZE of "
+                         "'bench.py --steps 2 --warmup 1 --no-cpu --no-gmg --no-elasticity "
+                         "--variants f64,f32' and of '--calibrate-copy N') into the traffic file")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the all-cores CPU run (0: the CPUs this process may use, "
                          "capped by OMP_NUM_THREADS)")
@@ -89,6 +93,41 @@ def color_pass_bytes(sm, n, ndc, s_v, first_reads_outer=True):
     total = sum(own + (outer if (k > 0 or first_reads_outer) else 0.0)
                 for k, (own, outer) in enumerate(per_color[c] for c in seq))
     return total / len(seq)
+
+
+def write_traffic(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, n=64, p=3):
+    """HBM bytes per launch of the fused color pass: FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch, mean over the dispatches of the kernel), each scaled by
+    what the counter reads for the copy kernel k_convert<float, double> (8N bytes
+    read, 4N written; N from its grid size)."""
+    import csv
+
+    def means(path):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            acc.setdefault(r["Kernel_Name"], []).append((float(r["Counter_Value"]),
+                                                         int(r["Grid_Size"])))
+        return acc
+
+    cf, cw = means(cal_fetch_csv), means(cal_write_csv)
+    copy = next(k for k in cf if "k_convert<float, double>" in k)
+    n_copy = max(g for _, g in cf[copy])
+    fetch_factor = 8.0 * n_copy / 1024.0 / np.mean([v for v, _ in cf[copy]])
+    write_factor = 4.0 * n_copy / 1024.0 / np.mean([v for v, _ in cw[copy]])
+    f, w = means(fetch_csv), means(write_csv)
+    out = {"fetch_factor": fetch_factor, "write_factor": write_factor,
+           "calibration": f"k_convert<float, double>, N = {n_copy}"}
+    for name, tag in (("f64", "double, double, 0>"), ("f32", "float, float, 0>")):
+        k = next((k for k in f if "k_patch_mvp_fused<3, 8, " + tag in k), None)
+        if k is None:
+            continue
+        fk, wk = float(np.mean([v for v, _ in f[k]])), float(np.mean([v for v, _ in w[k]]))
+        out[f"{name}_{n}_{p}"] = {"kernel": k.split("(")[0], "dispatches": len(f[k]),
+                                  "fetch_kib": fk, "write_kib": wk,
+                                  "bytes_per_launch": 1024.0 * (fetch_factor * fk + write_factor * wk)}
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
 
 
 def host_cpus():
@@ -243,6 +282,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the smoother path is single-GPU; see the sharding design note).", file=sys.stderr)
         sys.exit(2)
+    if args.write_traffic:
+        write_traffic(*args.write_traffic, n=args.n, p=args.degree)
+        return
     import tpss_amd as t
 
     if args.calibrate_copy:

@@ -993,6 +993,14 @@ static void operator_cartesian(const or_ctx *c, double *dst, const double *src)
   }
 }
 
+/* Gauss-Lobatto nodes and QGauss(p+1) of the moved-vertex operator */
+typedef struct
+{
+  double nodes[OR_MAXNP], qx[OR_MAXNP], qw[OR_MAXNP];
+  int nq, nqc, nqf;
+  double gamma;
+} dist_quad;
+
 static void operator_distorted(const or_ctx *c, double *dst, const double *src);
 
 static void operator_elasticity(const or_ctx *c, double *dst, const double *src);
@@ -1010,23 +1018,58 @@ void or_operator_vmult(const or_ctx *c, double *dst, const double *src)
  * operator applied to unit vectors: the patch matrix the reference's
  * TensorProductMatrix "basic" state and the Kronecker SVD start from
  * (tensor_product_matrix.h:136-304, kroneckersvd.h:215-296). */
+static void operator_distorted_one_cell(const or_ctx *c, const dist_quad *g, long cell,
+                                        double *dst, const double *src);
+static void dist_quad_init(const or_ctx *c, dist_quad *g);
+
 void or_patch_local_matrix(const or_ctx *c, int patch, double *A)
 {
   const int ndp = c->ndp;
   uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * ndp * c->ncomp);
-  double *x = (double *)calloc(c->N, sizeof(double));
-  double *y = (double *)malloc(sizeof(double) * c->N);
   patch_dofs(c, c->vtx + patch * c->dim, idx);
-  for(int col = 0; col < ndp; ++col)
+  const int one_cell = c->vertices && c->ncomp == 1;
+  dist_quad g;
+  if(one_cell) dist_quad_init(c, &g);
+#pragma omp parallel num_threads(one_cell ? c->n_threads : 1)
   {
-    x[idx[col]] = 1.0;
-    or_operator_vmult(c, y, x);
-    x[idx[col]] = 0.0;
-    for(int row = 0; row < ndp; ++row) A[(long)row * ndp + col] = y[idx[row]];
+    double *x = (double *)calloc(c->N, sizeof(double));
+    double *y = (double *)calloc(c->N, sizeof(double));
+#pragma omp for schedule(static)
+    for(int col = 0; col < ndp; ++col)
+    {
+      x[idx[col]] = 1.0;
+      if(one_cell)
+      {
+        /* a unit vector lives in one cell: only that cell and its faces
+         * contribute (and only to the rows of its face neighbours) */
+        const long cell = (long)(idx[col] / (uint32_t)c->ndc);
+        for(int row = 0; row < ndp; ++row) y[idx[row]] = 0.0;
+        operator_distorted_one_cell(c, &g, cell, y, x);
+      }
+      else
+        or_operator_vmult(c, y, x);
+      x[idx[col]] = 0.0;
+      for(int row = 0; row < ndp; ++row) A[(long)row * ndp + col] = y[idx[row]];
+      if(one_cell)
+      {
+        /* clear what the one-cell apply wrote outside the patch rows */
+        int cc[3];
+        const long cell = (long)(idx[col] / (uint32_t)c->ndc);
+        long r = cell, stride = 1;
+        for(int d = 0; d < c->dim; ++d) { cc[d] = (int)(r % c->n); r /= c->n; }
+        for(int d = 0; d < c->dim; ++d)
+        {
+          if(cc[d] > 0) memset(y + (cell - stride) * c->ndc, 0, sizeof(double) * c->ndc);
+          if(cc[d] < c->n - 1) memset(y + (cell + stride) * c->ndc, 0, sizeof(double) * c->ndc);
+          stride *= c->n;
+        }
+        memset(y + cell * c->ndc, 0, sizeof(double) * c->ndc);
+      }
+    }
+    free(x);
+    free(y);
   }
   free(idx);
-  free(x);
-  free(y);
 }
 
 /*
@@ -1610,178 +1653,220 @@ static void cell_shape(const or_ctx *c, const double *nodes, const double *xi, d
  * Dirichlet face  sigma u v - d_n u v - u d_n v, sigma = ip * 2 gamma / h.
  * h = 1 / |n^T J^{-1} e_normal| of each cell (laplace_integrator.h:651-656).
  */
-static void operator_distorted(const or_ctx *c, double *dst, const double *src)
+static void dist_quad_init(const or_ctx *c, dist_quad *g)
 {
-  const int dim = c->dim, np = c->np, nq = c->np, n = c->n, ndc = c->ndc;
-  double nodes[OR_MAXNP], qx[OR_MAXNP], qw[OR_MAXNP];
-  or_gauss_lobatto(np, nodes);
-  or_gauss_legendre(nq, qx, qw);
-  const int nqc = ipow(nq, dim), nqf = ipow(nq, dim - 1);
-  const double gamma = c->p == 0 ? 1.0 : (double)c->p * (c->p + 1);
-  memset(dst, 0, sizeof(double) * c->N);
-  /* cell terms */
-#pragma omp parallel for num_threads(c->n_threads) schedule(static)
-  for(long cell = 0; cell < c->ncell; ++cell)
+  g->nq = c->np;
+  or_gauss_lobatto(c->np, g->nodes);
+  or_gauss_legendre(g->nq, g->qx, g->qw);
+  g->nqc = ipow(g->nq, c->dim);
+  g->nqf = ipow(g->nq, c->dim - 1);
+  g->gamma = c->p == 0 ? 1.0 : (double)c->p * (c->p + 1);
+}
+
+static void cell_coords(const or_ctx *c, long cell, int *cc)
+{
+  long r = cell;
+  for(int d = 0; d < c->dim; ++d) { cc[d] = (int)(r % c->n); r /= c->n; }
+}
+
+/* cell term  int grad u . grad v  of one cell */
+static void cell_term_distorted(const or_ctx *c, const dist_quad *g, long cell, double *dst,
+                                const double *src)
+{
+  const int dim = c->dim, nq = g->nq, ndc = c->ndc;
+  int cc[3];
+  cell_coords(c, cell, cc);
+  double phi[512], dphi[512 * 3], X[3], J[9], Ji[9];
+  const double *u = src + cell * ndc;
+  double *y = dst + cell * ndc;
+  for(int q = 0; q < g->nqc; ++q)
   {
-    int cc[3];
-    long r = cell;
-    for(int d = 0; d < dim; ++d) { cc[d] = (int)(r % n); r /= n; }
-    double phi[512], dphi[512 * 3], X[3], J[9], Ji[9];
-    const double *u = src + cell * ndc;
-    double *y = dst + cell * ndc;
-    for(int q = 0; q < nqc; ++q)
-    {
-      double xi[3], w = 1.0;
-      int t = q;
-      for(int e = 0; e < dim; ++e) { xi[e] = qx[t % nq]; w *= qw[t % nq]; t /= nq; }
-      cell_map(c, cc, xi, X, J);
-      const double det = inv_det(dim, J, Ji);
-      cell_shape(c, nodes, xi, phi, dphi);
-      double gu[3] = {0, 0, 0};
-      for(int i = 0; i < ndc; ++i)
-        for(int a = 0; a < dim; ++a)
-        {
-          double g = 0.0;
-          for(int b = 0; b < dim; ++b) g += dphi[i * dim + b] * Ji[b * dim + a];
-          gu[a] += u[i] * g;
-        }
-      const double JxW = fabs(det) * w;
-      for(int i = 0; i < ndc; ++i)
+    double xi[3], w = 1.0;
+    int t = q;
+    for(int e = 0; e < dim; ++e) { xi[e] = g->qx[t % nq]; w *= g->qw[t % nq]; t /= nq; }
+    cell_map(c, cc, xi, X, J);
+    const double det = inv_det(dim, J, Ji);
+    cell_shape(c, g->nodes, xi, phi, dphi);
+    double gu[3] = {0, 0, 0};
+    for(int i = 0; i < ndc; ++i)
+      for(int a = 0; a < dim; ++a)
       {
-        double s = 0.0;
-        for(int a = 0; a < dim; ++a)
-        {
-          double g = 0.0;
-          for(int b = 0; b < dim; ++b) g += dphi[i * dim + b] * Ji[b * dim + a];
-          s += g * gu[a];
-        }
-        y[i] += s * JxW;
+        double gg = 0.0;
+        for(int b = 0; b < dim; ++b) gg += dphi[i * dim + b] * Ji[b * dim + a];
+        gu[a] += u[i] * gg;
       }
+    const double JxW = fabs(det) * w;
+    for(int i = 0; i < ndc; ++i)
+    {
+      double sacc = 0.0;
+      for(int a = 0; a < dim; ++a)
+      {
+        double gg = 0.0;
+        for(int b = 0; b < dim; ++b) gg += dphi[i * dim + b] * Ji[b * dim + a];
+        sacc += gg * gu[a];
+      }
+      y[i] += sacc * JxW;
     }
   }
-  /* faces: loop over cells and their "upper" faces (interior) plus every
-   * boundary face; serial to avoid write conflicts */
-  for(long cell = 0; cell < c->ncell; ++cell)
+}
+
+/* face (d, side) of a cell: side 1 = upper face (interior face shared with the
+ * upper neighbour, or boundary), side 0 = lower face, boundary only (an
+ * interior lower face is the upper face of the lower neighbour) */
+static void face_term_distorted(const or_ctx *c, const dist_quad *g, long cell, int d, int side,
+                                double *dst, const double *src)
+{
+  const int dim = c->dim, nq = g->nq, n = c->n, ndc = c->ndc;
+  int cc[3];
+  cell_coords(c, cell, cc);
+  long stride = 1;
+  for(int e = 0; e < d; ++e) stride *= n;
+  const int interior = side == 1 ? cc[d] < n - 1 : 0;
+  const int boundary = side == 0 ? cc[d] == 0 : cc[d] == n - 1;
+  if(!interior && !boundary) return;
+  const long nb = interior ? cell + stride : -1;
+  int cn[3];
+  for(int e = 0; e < dim; ++e) cn[e] = cc[e];
+  if(interior) cn[d] += 1;
+  double phim[512], dphim[512 * 3], phip[512], dphip[512 * 3];
+  double Xm[3], Jm[9], Jim[9], Xp[3], Jp[9], Jip[9];
+  /* the penalty uses h = 1/|(J^{-1} n)_d| of both cells at the first face
+   * quadrature point only (normal_vector(0) * inverse_jacobian(0),
+   * apps/laplace_integrator.h:651-656, 692-693) */
+  double hm0, hp0 = 1.0;
   {
-    int cc[3];
-    long r = cell;
-    for(int d = 0; d < dim; ++d) { cc[d] = (int)(r % n); r /= n; }
-    long stride = 1;
-    for(int d = 0; d < dim; ++d)
+    double xim[3], xip[3], nv[3], nn = 0.0, sq = 0.0;
+    for(int e = 0; e < dim; ++e)
     {
-      for(int side = 0; side < 2; ++side)
-      {
-        const int interior = side == 1 ? cc[d] < n - 1 : 0;
-        const int boundary = side == 0 ? cc[d] == 0 : cc[d] == n - 1;
-        if(!interior && !boundary) continue;
-        const long nb = interior ? cell + stride : -1;
-        int cn[3];
-        for(int e = 0; e < dim; ++e) cn[e] = cc[e];
-        if(interior) cn[d] += 1;
-        double phim[512], dphim[512 * 3], phip[512], dphip[512 * 3];
-        double Xm[3], Jm[9], Jim[9], Xp[3], Jp[9], Jip[9];
-        /* the penalty uses h = 1/|(J^{-1} n)_d| of both cells at the first
-         * face quadrature point only (normal_vector(0) * inverse_jacobian(0),
-         * apps/laplace_integrator.h:651-656, 692-693) */
-        double hm0, hp0 = 1.0;
-        {
-          double xim[3], xip[3], nv[3], nn = 0.0, s = 0.0;
-          for(int e = 0; e < dim; ++e)
-          {
-            xim[e] = e == d ? (double)side : qx[0];
-            xip[e] = e == d ? 0.0 : qx[0];
-          }
-          cell_map(c, cc, xim, Xm, Jm);
-          inv_det(dim, Jm, Jim);
-          for(int a = 0; a < dim; ++a) { nv[a] = Jim[d * dim + a] * (side == 0 ? -1.0 : 1.0); nn += nv[a] * nv[a]; }
-          nn = sqrt(nn);
-          for(int a = 0; a < dim; ++a) { nv[a] /= nn; s += nv[a] * Jim[d * dim + a]; }
-          hm0 = 1.0 / fabs(s);
-          if(interior)
-          {
-            cell_map(c, cn, xip, Xp, Jp);
-            inv_det(dim, Jp, Jip);
-            s = 0.0;
-            for(int a = 0; a < dim; ++a) s += nv[a] * Jip[d * dim + a];
-            hp0 = 1.0 / fabs(s);
-          }
-        }
-        for(int q = 0; q < nqf; ++q)
-        {
-          double xim[3], xip[3], w = 1.0;
-          int t = q;
-          for(int e = 0; e < dim; ++e)
-          {
-            if(e == d) { xim[e] = (double)side; xip[e] = 0.0; continue; }
-            xim[e] = xip[e] = qx[t % nq];
-            w *= qw[t % nq];
-            t /= nq;
-          }
-          cell_map(c, cc, xim, Xm, Jm);
-          const double detm = inv_det(dim, Jm, Jim);
-          (void)detm;
-          cell_shape(c, nodes, xim, phim, dphim);
-          /* outward normal of the "-" cell: n ~ J^{-T} e_d * sign; surface
-           * element = |det J| * |J^{-T} e_d| */
-          double nv[3], nn = 0.0;
-          for(int a = 0; a < dim; ++a) { nv[a] = Jim[d * dim + a] * (side == 0 ? -1.0 : 1.0); nn += nv[a] * nv[a]; }
-          nn = sqrt(nn);
-          for(int a = 0; a < dim; ++a) nv[a] /= nn;
-          const double dS = fabs(detm) * nn * w;
-          const double hm = hm0;
-          double um = 0.0, dnum = 0.0;
-          double gm[512];
-          for(int i = 0; i < ndc; ++i)
-          {
-            double g = 0.0;
-            for(int a = 0; a < dim; ++a)
-            {
-              double ga = 0.0;
-              for(int b = 0; b < dim; ++b) ga += dphim[i * dim + b] * Jim[b * dim + a];
-              g += ga * nv[a];
-            }
-            gm[i] = g;
-            um += phim[i] * src[cell * ndc + i];
-            dnum += g * src[cell * ndc + i];
-          }
-          if(boundary)
-          {
-            const double sigma = c->ip * 2.0 * gamma / hm;
-            for(int i = 0; i < ndc; ++i)
-              dst[cell * ndc + i] += (sigma * um * phim[i] - dnum * phim[i] - um * gm[i]) * dS;
-            continue;
-          }
-          cell_map(c, cn, xip, Xp, Jp);
-          inv_det(dim, Jp, Jip);
-          cell_shape(c, nodes, xip, phip, dphip);
-          const double hp = hp0;
-          double up = 0.0, dnup = 0.0;
-          double gp[512];
-          for(int i = 0; i < ndc; ++i)
-          {
-            double g = 0.0;
-            for(int a = 0; a < dim; ++a)
-            {
-              double ga = 0.0;
-              for(int b = 0; b < dim; ++b) ga += dphip[i * dim + b] * Jip[b * dim + a];
-              g += ga * nv[a];
-            }
-            gp[i] = g;
-            up += phip[i] * src[nb * ndc + i];
-            dnup += g * src[nb * ndc + i];
-          }
-          const double sigma = c->ip * 0.5 * or_penalty(c->p, hm, hp);
-          const double jump = um - up, avg = 0.5 * (dnum + dnup);
-          const double tv = (jump * sigma - avg) * dS, tg = -0.5 * jump * dS;
-          for(int i = 0; i < ndc; ++i)
-          {
-            dst[cell * ndc + i] += tv * phim[i] + tg * gm[i];
-            dst[nb * ndc + i] += -tv * phip[i] + tg * gp[i];
-          }
-        }
-      }
-      stride *= n;
+      xim[e] = e == d ? (double)side : g->qx[0];
+      xip[e] = e == d ? 0.0 : g->qx[0];
     }
+    cell_map(c, cc, xim, Xm, Jm);
+    inv_det(dim, Jm, Jim);
+    for(int a = 0; a < dim; ++a) { nv[a] = Jim[d * dim + a] * (side == 0 ? -1.0 : 1.0); nn += nv[a] * nv[a]; }
+    nn = sqrt(nn);
+    for(int a = 0; a < dim; ++a) { nv[a] /= nn; sq += nv[a] * Jim[d * dim + a]; }
+    hm0 = 1.0 / fabs(sq);
+    if(interior)
+    {
+      cell_map(c, cn, xip, Xp, Jp);
+      inv_det(dim, Jp, Jip);
+      sq = 0.0;
+      for(int a = 0; a < dim; ++a) sq += nv[a] * Jip[d * dim + a];
+      hp0 = 1.0 / fabs(sq);
+    }
+  }
+  for(int q = 0; q < g->nqf; ++q)
+  {
+    double xim[3], xip[3], w = 1.0;
+    int t = q;
+    for(int e = 0; e < dim; ++e)
+    {
+      if(e == d) { xim[e] = (double)side; xip[e] = 0.0; continue; }
+      xim[e] = xip[e] = g->qx[t % nq];
+      w *= g->qw[t % nq];
+      t /= nq;
+    }
+    cell_map(c, cc, xim, Xm, Jm);
+    const double detm = inv_det(dim, Jm, Jim);
+    cell_shape(c, g->nodes, xim, phim, dphim);
+    /* outward normal of the "-" cell: n ~ J^{-T} e_d * sign; surface
+     * element = |det J| * |J^{-T} e_d| */
+    double nv[3], nn = 0.0;
+    for(int a = 0; a < dim; ++a) { nv[a] = Jim[d * dim + a] * (side == 0 ? -1.0 : 1.0); nn += nv[a] * nv[a]; }
+    nn = sqrt(nn);
+    for(int a = 0; a < dim; ++a) nv[a] /= nn;
+    const double dS = fabs(detm) * nn * w;
+    double um = 0.0, dnum = 0.0;
+    double gm[512];
+    for(int i = 0; i < ndc; ++i)
+    {
+      double gsum = 0.0;
+      for(int a = 0; a < dim; ++a)
+      {
+        double ga = 0.0;
+        for(int b = 0; b < dim; ++b) ga += dphim[i * dim + b] * Jim[b * dim + a];
+        gsum += ga * nv[a];
+      }
+      gm[i] = gsum;
+      um += phim[i] * src[cell * ndc + i];
+      dnum += gsum * src[cell * ndc + i];
+    }
+    if(boundary)
+    {
+      const double sigma = c->ip * 2.0 * g->gamma / hm0;
+      for(int i = 0; i < ndc; ++i)
+        dst[cell * ndc + i] += (sigma * um * phim[i] - dnum * phim[i] - um * gm[i]) * dS;
+      continue;
+    }
+    cell_map(c, cn, xip, Xp, Jp);
+    inv_det(dim, Jp, Jip);
+    cell_shape(c, g->nodes, xip, phip, dphip);
+    double up = 0.0, dnup = 0.0;
+    double gp[512];
+    for(int i = 0; i < ndc; ++i)
+    {
+      double gsum = 0.0;
+      for(int a = 0; a < dim; ++a)
+      {
+        double ga = 0.0;
+        for(int b = 0; b < dim; ++b) ga += dphip[i * dim + b] * Jip[b * dim + a];
+        gsum += ga * nv[a];
+      }
+      gp[i] = gsum;
+      up += phip[i] * src[nb * ndc + i];
+      dnup += gsum * src[nb * ndc + i];
+    }
+    const double sigma = c->ip * 0.5 * or_penalty(c->p, hm0, hp0);
+    const double jump = um - up, avg = 0.5 * (dnum + dnup);
+    const double tv = (jump * sigma - avg) * dS, tg = -0.5 * jump * dS;
+    for(int i = 0; i < ndc; ++i)
+    {
+      dst[cell * ndc + i] += tv * phim[i] + tg * gm[i];
+      dst[nb * ndc + i] += -tv * phip[i] + tg * gp[i];
+    }
+  }
+}
+
+/*
+ * Matrix-free SIPG on the trilinear mesh with QGauss(p+1) on cells and faces
+ * (apps/laplace_integrator.h:613-743): cell  int grad u . grad v;
+ * interior face  sigma[[u]][[v]] - {d_n u}[[v]] - [[u]]{d_n v} with
+ * sigma = ip * 0.5 * 2 gamma (1/2 h_in^-1 + 1/2 h_out^-1);
+ * Dirichlet face  sigma u v - d_n u v - u d_n v, sigma = ip * 2 gamma / h.
+ * h = 1 / |n^T J^{-1} e_normal| of each cell (laplace_integrator.h:651-656).
+ */
+static void operator_distorted(const or_ctx *c, double *dst, const double *src)
+{
+  dist_quad g;
+  dist_quad_init(c, &g);
+  memset(dst, 0, sizeof(double) * c->N);
+#pragma omp parallel for num_threads(c->n_threads) schedule(static)
+  for(long cell = 0; cell < c->ncell; ++cell) cell_term_distorted(c, &g, cell, dst, src);
+  /* faces: every cell's upper faces plus its lower boundary faces; serial to
+   * avoid write conflicts */
+  for(long cell = 0; cell < c->ncell; ++cell)
+    for(int d = 0; d < c->dim; ++d)
+      for(int side = 0; side < 2; ++side) face_term_distorted(c, &g, cell, d, side, dst, src);
+}
+
+/* A src for src supported in one cell only: the cell term of that cell and
+ * the terms of its faces (the rows of the cell and of its face neighbours) */
+static void operator_distorted_one_cell(const or_ctx *c, const dist_quad *g, long cell,
+                                        double *dst, const double *src)
+{
+  cell_term_distorted(c, g, cell, dst, src);
+  int cc[3];
+  cell_coords(c, cell, cc);
+  long stride = 1;
+  for(int d = 0; d < c->dim; ++d)
+  {
+    face_term_distorted(c, g, cell, d, 1, dst, src);
+    if(cc[d] == 0)
+      face_term_distorted(c, g, cell, d, 0, dst, src);
+    else
+      face_term_distorted(c, g, cell - stride, d, 1, dst, src);
+    stride *= c->n;
   }
 }
 

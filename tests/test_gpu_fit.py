@@ -11,17 +11,38 @@ from test_gpu_parity import _tpss, dev, host, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def make_fit(dim, p, n, variant, amp, seed=3, sweeps=4, ip=4.0):
+def make_fit(dim, p, n, variant, amp, seed=3, sweeps=4, ip=4.0, precision=0):
     t = _tpss()
     V = None if amp == 0 else o.distorted_vertices(dim, n, amp, seed=seed)
     coloring = t.COLOR_NONE if variant == t.ADDITIVE else t.COLOR_REDBLACK
     relax = 0.7 if variant == t.ADDITIVE else 1.0
-    lv = t.Level(dim, p, n, ip_factor=ip, vertices=V)
+    lv = t.Level(dim, p, n, ip_factor=ip, vertices=V, precision=precision)
     sm = t.SchwarzSmoother(lv, variant=variant, coloring=coloring, relaxation=relax,
                            local_solver=t.LOCAL_FIT, fit_sweeps=sweeps)
     ref = o.Smoother(dim, p, n, ip=ip, vertices=V, variant=variant, coloring=coloring,
-                     relaxation=relax, local_solver=o.LOCAL_FIT, fit_sweeps=sweeps)
+                     relaxation=relax, local_solver=o.LOCAL_FIT, fit_sweeps=sweeps, threads=8)
     return lv, sm, ref
+
+
+@pytest.mark.parametrize("n,variant", [(3, 0), (3, 1), (4, 0)])
+def test_fit_parity_q4_configuration_3(n, variant):
+    """Configuration 3's local solver: 3D DG Q4 (k = 10, 1000 DoFs per patch), interior
+    vertices moved by 0.15 h, rank-2 separable fit of the exact local matrices; the
+    additive smoother with damping 0.7 and the multiplicative one, FP64 and FP32 patch
+    solves (the FP32 smoother keeps the FP64 setup)."""
+    t = _tpss()
+    lv, sm, ref = make_fit(3, 4, n, variant, 0.15)
+    assert ref.fit_fallbacks() == 0
+    np.testing.assert_allclose(sm.fit_errors(), ref.fit_errors(), rtol=1e-9, atol=1e-14)
+    src = o.splitmix_uniform(3, ref.N)
+    d = dev(np.zeros(ref.N))
+    sm.precond_vmult(d, dev(src))
+    expect = ref.precond_vmult(src)
+    assert rel_err(host(d), expect) < 1e-10
+    lv32, sm32, _ = make_fit(3, 4, n, variant, 0.15, precision=t.F32)
+    d32 = dev(np.zeros(ref.N, dtype=np.float32))
+    sm32.precond_vmult(d32, dev(src.astype(np.float32)))
+    assert rel_err(host(d32), expect) < 2e-5
 
 
 @pytest.mark.parametrize("dim,p,n,variant,amp", [(3, 1, 4, 0, 0.2), (3, 2, 3, 0, 0.15),

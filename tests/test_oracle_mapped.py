@@ -135,3 +135,17 @@ def test_sheared_energy_closed_form(dim, p, n):
     side = np.linalg.norm(M[:, 0]) / n
     assert abs(side - abs(np.linalg.det(M)) / (n * np.linalg.norm(M[:, 1] if dim == 2 else
                                                                    np.cross(M[:, 1], M[:, 2])))) > 1e-3
+
+
+@pytest.mark.parametrize("dim,p,n", [(2, 2, 3), (3, 1, 3), (3, 2, 2)])
+def test_local_matrix_is_the_submatrix_of_the_dense_assembly(dim, p, n):
+    """or_patch_local_matrix (unit vectors applied cell by cell on moved-vertex
+    meshes) == R_j A R_j^T of the independent dense assembly"""
+    V = o.distorted_vertices(dim, n, 0.25, seed=6)
+    A = assemble_mapped(dim, p, n, 3.0, V)
+    sm = o.Smoother(dim, p, n, ip=3.0, vertices=V)
+    idx = sm.patch_dof_indices().astype(np.int64)
+    for j in range(sm.P):
+        Aj = sm.patch_local_matrix(j)
+        ref = A[np.ix_(idx[j], idx[j])]
+        np.testing.assert_allclose(Aj, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
