@@ -155,6 +155,20 @@ int tpss_precond_vmult_add(tpss_smoother *sm, void *dst, const void *src, void *
 int tpss_precond_Tvmult(tpss_smoother *sm, void *dst, const void *src, void *stream);
 /* SchwarzPreconditioner::Tvmult_add (templates.h:197-206): dst += P^T src */
 int tpss_precond_Tvmult_add(tpss_smoother *sm, void *dst, const void *src, void *stream);
+/* Allocation-free variant of tpss_precond_vmult.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The scratch
+ * the preconditioner needs (the additive slot buffer, or the residual of
+ * multiplicative sweeps that update it) comes from one file-scope device array
+ * of TPSS_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends;
+ * smoothers that need more are rejected with TPSS_E_INVALID_ARG.  The call
+ * allocates nothing (no hipMalloc / hipFree), records no section timers, and
+ * shares the array with every other such call on the GPU: calls that overlap
+ * on one GPU race.  tpss_noalloc_init looks up the array's device address for
+ * the current device (otherwise the first call on a device does). */
+#define TPSS_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tpss_noalloc_init(void);
+int tpss_precond_vmult_noalloc(tpss_smoother *sm, void *dst, const void *src, void *stream);
 /* SmootherRichardson::step (smoother_richardson.h:118-145) */
 int tpss_smoother_step(tpss_smoother *sm, void *dst, const void *src, void *stream);
 /* SmootherRichardson::vmult (smoother_richardson.h:147-156) */

@@ -78,7 +78,8 @@ EXPORTS = [
     "tpss_level_n_dofs", "tpss_operator_vmult", "tpss_operator_residual",
     "tpss_smoother_create", "tpss_smoother_clone_shallow", "tpss_smoother_destroy",
     "tpss_precond_vmult", "tpss_precond_vmult_add", "tpss_precond_Tvmult",
-    "tpss_precond_Tvmult_add", "tpss_smoother_step", "tpss_smoother_vmult",
+    "tpss_precond_Tvmult_add", "tpss_noalloc_init", "tpss_precond_vmult_noalloc",
+    "tpss_smoother_step", "tpss_smoother_vmult",
     "tpss_smoother_set_patch_damping", "tpss_smoother_info", "tpss_smoother_color_ptr",
     "tpss_smoother_patch_vertices", "tpss_smoother_patch_dof_indices", "tpss_smoother_patch_h",
     "tpss_smoother_restricted_dof_flags",
@@ -123,6 +124,8 @@ def _load():
         "tpss_precond_vmult_add": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tpss_precond_Tvmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tpss_precond_Tvmult_add": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tpss_noalloc_init": (ctypes.c_int, []),
+        "tpss_precond_vmult_noalloc": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tpss_smoother_step": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tpss_smoother_vmult": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tpss_smoother_set_patch_damping": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),

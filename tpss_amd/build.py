@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtpss_amd.so")
 
-HIP_SOURCES = ["capi.hip", "smoother.hip", "gmg.hip", "mlp.hip", "ksvd.hip", "kron.hip"]
+HIP_SOURCES = ["capi.hip", "smoother.hip", "gmg.hip", "mlp.hip", "ksvd.hip", "kron.hip", "noalloc.hip"]
 CXX_SOURCES = ["basis.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
