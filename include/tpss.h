@@ -283,6 +283,11 @@ int tpss_smoother_patch_features(const tpss_smoother *sm, float *dev_out, void *
  * (d_j = R_j^T A_j^{-1} R_j r, r = A e, e random), clamped (device float[n_patches]) */
 int tpss_smoother_optimal_damping(tpss_smoother *sm, int n_samples, uint64_t seed,
                                   double omega_min, double omega_max, float *dev_out);
+/* per-patch largest eigenvalue of M_j^{-1} A_j (M_j: the patch's local solver,
+ * A_j = R_j A R_j^T the exact local matrix; 1 for exact local solvers) by
+ * n_iter power iterations in the A_j inner product, the patches of one
+ * red-black color at a time (device double[n_patches]; FP64 Poisson levels) */
+int tpss_smoother_local_lambda_max(tpss_smoother *sm, int n_iter, uint64_t seed, double *dev_lambda);
 /* omega_j = MLP(features_j) for every patch (NULL restores the scalar relaxation) */
 int tpss_smoother_set_damping_mlp(tpss_smoother *sm, tpss_mlp *m);
 

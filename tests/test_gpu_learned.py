@@ -88,27 +88,58 @@ def test_training_reduces_loss_and_sets_damping():
     assert np.abs(out - expect).max() < 1e-12 * np.abs(expect).max()
 
 
-@pytest.mark.parametrize("calibrated,n", [(False, 16), (True, 32)])
-def test_shipped_model_beats_fixed_damping(calibrated, n):
-    """The trained network (tpss_amd/models/damping_dgq3_3d.npy), with its own
-    prediction (16^3 cells) and with the output scale calibrated on a 64^3
-    training mesh (32^3 cells), against the reference's fixed multiplicative
-    damping 1 (TPSS.h:120-163): GMG-CG on a distorted DG Q3 mesh (interior
-    vertices moved by 0.2 h, seed 3) contracts faster and needs fewer iterations
-    to 1e-8."""
+@pytest.mark.parametrize("pattern", ["half", "uniform", "graded"])
+def test_shipped_model_beats_the_best_constant_damping(pattern):
+    """The trained network (tpss_amd/models/damping_dgq3_3d.npy) as smoother damping,
+    omega_j = 1 - g (1 - w_j) with the gain g calibrated on other meshes, against every
+    constant damping 1.0, 0.9, ..., 0.6 swept on the evaluation mesh itself: 3D DG Q3 on
+    64^3 cells, interior vertices moved by 0.2 h (seed 3, neither a training nor a
+    calibration mesh) everywhere, growing with x, or only where x > 1/2 (the other
+    half Cartesian); mixed-precision GMG-CG to 1e-8.  The network's V-cycle contracts
+    faster than the best constant's on every pattern and, on the half-distorted mesh,
+    needs one CG iteration less."""
     t = _tpss()
     from tpss_amd import learned_damping as ld
-    mlp = ld.load(scale=ld.OUTPUT_SCALE if calibrated else 1.0)
-    V = ld.distorted_vertices(3, n, 0.2, seed=3)
-    levels = int(np.log2(n // 2)) + 1
-    res = {}
-    for use in (False, True):
-        g = t.GMGSolver(3, 3, 2, levels, ip_factor=4.0, vertices=V, variant=t.MULTIPLICATIVE,
-                        relaxation=1.0)
-        if use:
-            ld.apply_to_gmg(g, mlp)
-        b = o.splitmix_uniform(2, g.n_dofs)
-        x = t.DeviceVector.zeros(g.n_dofs)
-        info, hist = g.solve(x, t.DeviceVector.from_numpy(b), rel_tol=1e-8, max_iter=100)
-        res[use] = (info.iterations, info.reduction_rate)
-    assert res[True][1] < 0.8 * res[False][1] and res[True][0] < res[False][0], res
+    n = 64
+    V = ld.distorted_vertices(3, n, 0.2, seed=3, pattern=pattern)
+    consts = {c: ld.gmg_solve(V, n, relaxation=c)[:2]
+              for c in (1.0, 0.9, 0.8, 0.75, 0.7, 0.65, 0.6)}
+    best_it = min(it for it, _ in consts.values() if it > 0)
+    best_rate = min(rate for _, rate in consts.values())
+    it, rate = ld.gmg_solve(V, n, mlp=ld.load())[:2]
+    assert 0 < it <= best_it and rate < 0.95 * best_rate, (it, rate, consts)
+    if pattern == "half":
+        assert it < best_it, (it, rate, consts)
+    # the reference's fixed damping 1 (TPSS.h:120-163) is far behind
+    assert consts[1.0][0] < 0 or consts[1.0][0] >= it + 3, consts[1.0]
+
+
+def test_local_lambda_max_matches_dense_eigenproblem():
+    """largest eigenvalue of M_j^{-1} A_j per patch (surrogate fast-diagonalization
+    solver M_j against the exact local matrix A_j of a moved-vertex patch) against
+    scipy's generalized symmetric eigensolver on the oracle's dense matrices;
+    exactly 1 on a Cartesian mesh, where the local solver is exact"""
+    import scipy.linalg
+
+    t = _tpss()
+    dim, p, n = 3, 2, 4
+    V = o.distorted_vertices(dim, n, 0.2, seed=3)
+    lv = t.Level(dim, p, n, ip_factor=4.0, vertices=V)
+    sm = t.SchwarzSmoother(lv, variant=t.MULTIPLICATIVE)
+    lam = sm.local_lambda_max(n_iter=300)
+    ref = o.Smoother(dim, p, n, ip=4.0, variant=o.MULTIPLICATIVE, coloring=o.COLOR_REDBLACK,
+                     vertices=V)
+    ndp = ref.ndp
+    for j in range(ref.P):
+        A = ref.patch_local_matrix(j)
+        Minv = np.stack([ref.local_apply(j, e) for e in np.eye(ndp)], axis=1)
+        M = np.linalg.inv(0.5 * (Minv + Minv.T))
+        ev = np.sort(scipy.linalg.eigh(A, 0.5 * (M + M.T), eigvals_only=True))
+        # the Rayleigh quotient approaches the largest eigenvalue from below, at a
+        # rate set by the gap to the next one
+        assert ev[-1] * (1 - 2e-3) < lam[j] < ev[-1] * (1 + 1e-9), (j, lam[j], ev[-2:])
+        if ev[-1] - ev[-2] > 0.05 * ev[-1]:
+            assert abs(lam[j] - ev[-1]) < 1e-8 * ev[-1], (j, lam[j], ev[-2:])
+    lvc = t.Level(dim, p, n, ip_factor=4.0)
+    smc = t.SchwarzSmoother(lvc, variant=t.MULTIPLICATIVE)
+    np.testing.assert_allclose(smc.local_lambda_max(n_iter=20), 1.0, rtol=1e-10)

@@ -95,6 +95,7 @@ EXPORTS = [
     "tpss_mlp_create", "tpss_mlp_destroy", "tpss_mlp_n_params", "tpss_mlp_set_params",
     "tpss_mlp_get_params", "tpss_mlp_forward", "tpss_mlp_loss_grad", "tpss_mlp_train",
     "tpss_smoother_patch_features", "tpss_smoother_optimal_damping", "tpss_smoother_set_damping_mlp",
+    "tpss_smoother_local_lambda_max",
     "tpss_ksvd_batched", "tpss_ksvd_sum_batched", "tpss_kron_vmult", "tpss_kron_table",
     "tpss_kron_pinv", "tpss_kcp_batched",
 ]
@@ -174,6 +175,7 @@ def _load():
         "tpss_smoother_optimal_damping": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64,
                                                          ctypes.c_double, ctypes.c_double, _VP]),
         "tpss_smoother_set_damping_mlp": (ctypes.c_int, [_VP, _VP]),
+        "tpss_smoother_local_lambda_max": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64, _VP]),
         "tpss_ksvd_batched": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP,
                                              _VP, _VP]),
@@ -425,6 +427,14 @@ class SchwarzPreconditioner:
         v = DeviceVector(self.n_patches, F32)
         _check(lib.tpss_smoother_optimal_damping(self.handle, int(n_samples), int(seed),
                                                  float(omega_min), float(omega_max), _ptr(v)))
+        return v.numpy()
+
+    def local_lambda_max(self, n_iter=30, seed=7):
+        """Largest eigenvalue of M_j^{-1} A_j per patch (local solver M_j against the
+        exact local matrix A_j; 1 for exact local solvers) by power iteration
+        (FP64 levels only)."""
+        v = DeviceVector(self.n_patches, F64)
+        _check(lib.tpss_smoother_local_lambda_max(self.handle, int(n_iter), int(seed), _ptr(v)))
         return v.numpy()
 
     def set_damping_mlp(self, mlp):

@@ -40,7 +40,9 @@ k_fit_unit(double *__restrict__ x, const uint32_t *__restrict__ vtx,
   x[patch_cell<DIM, NP>(v, n, i0, i1, i2) * NDC + patch_local<DIM, NP>(i0, i1, i2)] = val;
 }
 
-// A[jb][row][col] = y[global DoF of patch-local row] for every batch patch
+// A[jb][col][row] = y[global DoF of patch-local row] for every batch patch: the
+// local matrix is symmetric, so the column is stored as a row (the writes of one
+// launch are contiguous per patch instead of a stride of n_pd apart)
 template<int DIM, int K>
 __global__ void __launch_bounds__(256)
 k_fit_gather(const double *__restrict__ y, const uint32_t *__restrict__ vtx,
@@ -54,7 +56,7 @@ k_fit_gather(const double *__restrict__ y, const uint32_t *__restrict__ vtx,
   int v[3];
   unpack_vertex(vtx[plist[jb]], v);
   const int i0 = r % K, i1 = DIM > 1 ? (r / K) % K : 0, i2 = DIM > 2 ? r / (K * K) : 0;
-  A[(size_t(jb) * NPD + r) * NPD + col] =
+  A[(size_t(jb) * NPD + col) * NPD + r] =
     y[patch_cell<DIM, NP>(v, n, i0, i1, i2) * NDC + patch_local<DIM, NP>(i0, i1, i2)];
 }
 

@@ -430,6 +430,16 @@ __global__ void k_damping_from_ratio(const double *__restrict__ num, const doubl
   out[j] = float(w);
 }
 
+__global__ void k_lambda_from_ratio(const double *__restrict__ num, const double *__restrict__ den,
+                                    const int *__restrict__ ids, const int count,
+                                    double *__restrict__ out)
+{
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i >= count) return;
+  const int j = ids[i];
+  out[j] = den[j] > 0.0 ? num[j] / den[j] : 0.0;
+}
+
 int red_black_color(int dim, const int *v)
 {
   int mask = 0, sum = 0;
@@ -563,6 +573,74 @@ void optimal_damping(tpss_smoother *sm, int n_samples, uint64_t seed, double lo,
   hipLaunchKernelGGL(k_damping_from_ratio, dim3(blocks_for(ls.P, 256)), dim3(256), 0, s, num.ptr,
                      den.ptr, ls.P, lo, hi, dev_out);
   TPSS_HIP(hipGetLastError());
+  TPSS_HIP(hipStreamSynchronize(s));
+}
+// Largest eigenvalue of A_j^{-1}_local A_j (local solver vs exact local matrix)
+// per patch by power iteration in the A_j inner product: v <- M_j^{-1} A_j v,
+// lambda = (A v)^T M^{-1} (A v) / v^T A v (the Rayleigh quotient of M^{-1} A),
+// all patches of one red-black color at once (not face coupled, so
+// (A v)|_j = A_j v_j for v supported on the color's patches)
+void local_lambda_max(tpss_smoother *sm, int n_iter, uint64_t seed, double *dev_lambda,
+                      hipStream_t s)
+{
+  const LocalSolvers &ls = *sm->ls;
+  tpss_level *lv = sm->level;
+  TPSS_REQUIRE(lv->vec_prec == TPSS_F64, "local spectra need an FP64 level");
+  const int dim = ls.dim;
+  std::vector<std::vector<int>> groups(2 << dim);
+  for(int j = 0; j < ls.P; ++j)
+    groups[red_black_color(dim, &ls.vtx_unpacked[size_t(j) * dim])].push_back(j);
+  const int64_t N = lv->N;
+  DeviceArray<double> e{size_t(N)}, v{size_t(N)}, w{size_t(N)}, num{size_t(ls.P)},
+    den{size_t(ls.P)};
+  DeviceArray<int> gid;
+  hipLaunchKernelGGL((k_random<double>), dim3(blocks_for(N, 256)), dim3(256), 0, s, e.ptr, N,
+                     seed);
+  for(size_t c = 0; c < groups.size(); ++c)
+  {
+    const int count = int(groups[c].size());
+    if(count == 0) continue;
+    TPSS_HIP(hipStreamSynchronize(s));  // the previous group is done with gid
+    gid.upload(groups[c]);
+    dispatch_dim_np(dim, ls.np, [&](auto Dm, auto NPc) {
+      constexpr int DIM = decltype(Dm)::value, K = 2 * decltype(NPc)::value;
+      using PS = PatchShape<DIM, K>;
+      // start: v = M^{-1} e on the group's patches, zero elsewhere
+      fill_zero(v.ptr, TPSS_F64, N, s);
+      hipLaunchKernelGGL((k_patch_solve_ids<DIM, K, double>), dim3((count + PS::PPB - 1) / PS::PPB),
+                         dim3(PS::THREADS), 0, s, e.ptr, v.ptr, ls.d_vtx.ptr, ls.d_fac.ptr,
+                         ls.S64.ptr, ls.lam64.ptr, gid.ptr, count, ls.n);
+      // the iterates alternate between v and e, both zero off the group's patches
+      fill_zero(e.ptr, TPSS_F64, N, s);
+      for(int it = 0; it < n_iter; ++it)
+      {
+        operator_apply_any(lv, w.ptr, v.ptr, nullptr, nullptr, 0, s);
+        if(it + 1 == n_iter)
+        {
+          fill_zero(num.ptr, TPSS_F64, ls.P, s);
+          fill_zero(den.ptr, TPSS_F64, ls.P, s);
+        }
+        // v <- M^{-1} (A v) on the patches (the previous v's entries are overwritten;
+        // the rest of v stays zero)
+        hipLaunchKernelGGL((k_patch_solve_ids<DIM, K, double>),
+                           dim3((count + PS::PPB - 1) / PS::PPB), dim3(PS::THREADS), 0, s, w.ptr,
+                           e.ptr, ls.d_vtx.ptr, ls.d_fac.ptr, ls.S64.ptr, ls.lam64.ptr, gid.ptr,
+                           count, ls.n);
+        if(it + 1 == n_iter)
+          // num = w . M^{-1} w, den = w . v
+          hipLaunchKernelGGL((k_patch_dots<DIM, K, double>), dim3(count), dim3(64), 0, s, w.ptr,
+                             e.ptr, v.ptr, ls.d_vtx.ptr, gid.ptr, count, ls.n, num.ptr, den.ptr);
+        std::swap(v.ptr, e.ptr);
+        TPSS_HIP(hipGetLastError());
+      }
+      // e is overwritten by the iteration: a fresh random start for the next group
+      hipLaunchKernelGGL((k_random<double>), dim3(blocks_for(N, 256)), dim3(256), 0, s, e.ptr, N,
+                         seed + 1 + c);
+      hipLaunchKernelGGL(k_lambda_from_ratio, dim3(blocks_for(count, 256)), dim3(256), 0, s,
+                         num.ptr, den.ptr, gid.ptr, count, dev_lambda);
+      TPSS_HIP(hipGetLastError());
+    });
+  }
   TPSS_HIP(hipStreamSynchronize(s));
 }
 }  // namespace
@@ -717,6 +795,16 @@ int tpss_smoother_optimal_damping(tpss_smoother *sm, int n_samples, uint64_t see
     TPSS_REQUIRE(sm->level->ncomp == 1, "learned damping: Poisson levels only");
     TPSS_REQUIRE(omega_max > omega_min, "empty damping range");
     optimal_damping(sm, n_samples, seed, omega_min, omega_max, dev_out, nullptr);
+  });
+}
+
+int tpss_smoother_local_lambda_max(tpss_smoother *sm, int n_iter, uint64_t seed,
+                                   double *dev_lambda)
+{
+  return guarded([&] {
+    TPSS_REQUIRE(sm && dev_lambda && n_iter >= 1, "invalid arguments");
+    TPSS_REQUIRE(sm->level->ncomp == 1, "local spectra: Poisson levels only");
+    local_lambda_max(sm, n_iter, seed, dev_lambda, nullptr);
   });
 }
 

@@ -100,46 +100,76 @@ void fit_local_solvers(const tpss_level *lv, LocalSolvers *ls, int sweeps, const
   std::vector<std::vector<int>> by_color(2 << dim);
   for(int jp = 0; jp < P; ++jp)
     by_color[patch_color(dim, TPSS_COLOR_REDBLACK, &ls->vtx_unpacked[size_t(jp) * dim])].push_back(jp);
-  const size_t cap_bytes = size_t(4) << 30;
+  // local matrices of a batch: the more patches one operator launch covers, the
+  // fewer launches the npd columns take (a third of the free device memory, at
+  // most 48 GiB: one batch per color at 48^3 Q4 on a 288 GB device)
+  size_t free_bytes = 0, total_bytes = 0;
+  TPSS_HIP(hipMemGetInfo(&free_bytes, &total_bytes));
+  const size_t cap_bytes =
+    std::max<size_t>(size_t(1) << 30, std::min<size_t>(size_t(48) << 30, free_bytes / 3));
   const int batch = int(std::max<size_t>(1, std::min<size_t>(
     size_t(P), cap_bytes / (size_t(npd) * npd * sizeof(double)))));
   DeviceArray<double> x{size_t(lv->Ns)}, y{size_t(lv->Ns)};
   fill_zero(x.ptr, TPSS_F64, lv->Ns, s);
   DeviceArray<double> Abuf{size_t(std::min(batch, P)) * npd * npd};
   DeviceArray<int> d_plist;
-  DeviceArray<uint32_t> d_cells;
+  // per position cb of the unit vector's cell in the patch: that cell and its
+  // face neighbours inside the patch (the only patch cells where A e is not zero)
+  std::vector<DeviceArray<uint32_t>> d_cells(size_t(1) << dim);
+  const int ncb = 1 + dim;  // cells per patch in each list
   for(const auto &members : by_color)
     for(size_t b0 = 0; b0 < members.size(); b0 += size_t(batch))
     {
       const int nb = int(std::min(members.size() - b0, size_t(batch)));
       std::vector<int> plist(members.begin() + b0, members.begin() + b0 + nb);
-      std::vector<uint32_t> cells;
-      for(int jp : plist)
-        for(int cb = 0; cb < (1 << dim); ++cb)
+      auto cell_at = [&](int jp, int cb) {
+        int64_t c = 0, st = 1;
+        for(int d = 0; d < dim; ++d)
         {
-          int64_t c = 0, st = 1;
-          for(int d = 0; d < dim; ++d)
-          {
-            c += int64_t(ls->vtx_unpacked[size_t(jp) * dim + d] - 1 + ((cb >> d) & 1)) * st;
-            st *= n;
-          }
-          cells.push_back(uint32_t(c));
+          c += int64_t(ls->vtx_unpacked[size_t(jp) * dim + d] - 1 + ((cb >> d) & 1)) * st;
+          st *= n;
         }
+        return uint32_t(c);
+      };
       TPSS_HIP(hipStreamSynchronize(s));  // the previous batch is done with the lists
+      for(int cb = 0; cb < (1 << dim); ++cb)
+      {
+        std::vector<uint32_t> cells;
+        cells.reserve(size_t(nb) * ncb);
+        for(int jp : plist)
+        {
+          cells.push_back(cell_at(jp, cb));
+          for(int d = 0; d < dim; ++d) cells.push_back(cell_at(jp, cb ^ (1 << d)));
+        }
+        d_cells[cb].upload(cells);
+      }
       d_plist.upload(plist);
-      d_cells.upload(cells);
       dispatch_dim_np(dim, lv->np, [&](auto D, auto NPc) {
         constexpr int DIM = decltype(D)::value, KC = 2 * decltype(NPc)::value;
-        for(int col = 0; col < npd; ++col)
+        constexpr int NP = KC / 2, NDC = cpow(NP, DIM);
+        for(int cb = 0; cb < (1 << DIM); ++cb)
         {
-          hipLaunchKernelGGL((k_fit_unit<DIM, KC>), dim3(blocks_for(nb, 256)), dim3(256), 0, s,
-                             x.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col, 1.0);
-          operator_apply_f64(lv, y.ptr, x.ptr, d_cells.ptr, int64_t(cells.size()), s);
-          hipLaunchKernelGGL((k_fit_gather<DIM, KC>), dim3(blocks_for(int64_t(nb) * npd, 256)),
-                             dim3(256), 0, s, y.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col,
-                             Abuf.ptr);
-          hipLaunchKernelGGL((k_fit_unit<DIM, KC>), dim3(blocks_for(nb, 256)), dim3(256), 0, s,
-                             x.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col, 0.0);
+          // the patch cells outside the list keep A e = 0 for every column of this cb
+          fill_zero(y.ptr, TPSS_F64, lv->Ns, s);
+          for(int i = 0; i < NDC; ++i)
+          {
+            // patch-local index of local node i of the cell at position cb
+            int col = 0, st = 1, r = i;
+            for(int d = 0; d < DIM; ++d)
+            {
+              col += (((cb >> d) & 1) * NP + r % NP) * st;
+              r /= NP;
+              st *= KC;
+            }
+            hipLaunchKernelGGL((k_fit_unit<DIM, KC>), dim3(blocks_for(nb, 256)), dim3(256), 0, s,
+                               x.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col, 1.0);
+            operator_apply_f64(lv, y.ptr, x.ptr, d_cells[cb].ptr, int64_t(nb) * ncb, s);
+            hipLaunchKernelGGL((k_fit_gather<DIM, KC>), dim3(blocks_for(int64_t(nb) * npd, 256)),
+                               dim3(256), 0, s, y.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col,
+                               Abuf.ptr);
+            hipLaunchKernelGGL((k_fit_unit<DIM, KC>), dim3(blocks_for(nb, 256)), dim3(256), 0, s,
+                               x.ptr, ls->d_vtx.ptr, d_plist.ptr, nb, n, col, 0.0);
+          }
         }
         hipLaunchKernelGGL((k_separable_fit<DIM, KC>), dim3(nb), dim3(256), 0, s, Abuf.ptr,
                            d_plist.ptr, nb, Mf.ptr, Lf.ptr, sweeps, d_err.ptr);

@@ -15,10 +15,15 @@ constructions the distorted-mesh experiments use.
 import numpy as np
 
 
-def distorted_vertices(dim, n, amplitude, seed, graded=False):
+def distorted_vertices(dim, n, amplitude, seed, graded=False, pattern="uniform"):
     """Interior vertices moved by (amplitude/n) * U[-1,1]^dim; boundary vertices
-    fixed.  graded=True scales the amplitude by the vertex's x coordinate (no
-    distortion at x = 0, the full amplitude at x = 1)."""
+    fixed.  pattern "graded" (or graded=True) scales the amplitude by the vertex's
+    x coordinate (no distortion at x = 0, the full amplitude at x = 1); "half"
+    moves only the vertices with x > 1/2 (the other half of the mesh stays
+    Cartesian)."""
+    if pattern not in ("uniform", "graded", "half"):
+        raise ValueError("pattern must be uniform, graded or half")
+    graded = graded or pattern == "graded"
     g = np.linspace(0.0, 1.0, n + 1)
     mesh = np.meshgrid(*([g] * dim), indexing="ij")
     v = np.stack([m.transpose(tuple(range(dim))[::-1]).ravel() for m in mesh], axis=-1)
@@ -38,6 +43,8 @@ def distorted_vertices(dim, n, amplitude, seed, graded=False):
         interior &= (c > 0) & (c < n)
         t //= n + 1
     amp = amplitude * (v[:, :1] if graded else 1.0)
+    if pattern == "half":
+        amp = amplitude * (v[:, :1] > 0.5 + 1e-12)
     v[interior] += ((amp / n) * noise)[interior]
     return v
 
