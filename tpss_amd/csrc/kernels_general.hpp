@@ -14,15 +14,31 @@
 // from both sides, which keeps the kernel free of write races).
 //
 // Every contraction is sum-factorised, and independent work shares barriers:
-//   P0 load u, the face neighbours and the corners of the cell and neighbours
-//   P1 x-interpolation of u (value, derivative); map coefficients
+//   P0 load u, the face neighbours, the corners of the cell and the far corners
+//      of the face neighbours
+//   P1 x-interpolation of u (value, derivative); map coefficients of the cell;
+//      per face the tangent and normal-direction derivatives of both maps as
+//      polynomials in the face coordinates
 //   P2 y-interpolation; face traces along the normals (value, normal derivative)
 //   P3 z-interpolation -> reference gradient -> geometry -> reference flux;
 //      face traces along the first tangential direction
 //   P4 z-integration; face traces along the second tangential direction,
-//      face geometry (own and neighbour Jacobians) and fluxes per face point
+//      face geometry and fluxes per face point
 //   P5 y-integration; face fluxes projected on the tangential test functions
+//      (3D: sum-factorised, first along e1, then -- after a barrier -- along e2)
 //   P6 x-integration; add the face contributions of this test function
+//
+// Face geometry without 3x3 inverses.  The two cells of a face share the face's
+// vertices, so on the face their maps have the same tangent vectors t1, t2 (the
+// Jacobian columns of the tangential directions) and differ only in the column
+// J_d of the normal direction.  With w = t1 x t2 (2D: t1 turned by 90 degrees),
+// the outward unit normal is n = +-w/|w| (the sign makes n . J_d point away from
+// the cell), the surface element is |det J| |J^{-T} e_d| = |w|, and the
+// reference-gradient coefficients c = J^{-1} n of the normal derivative follow
+// from J c = n: c_d = 1/(J_d . n) (n is orthogonal to t1, t2), and the tangential
+// part solves the 2x2 Gram system G c_t = -c_d (t1 . J_d, t2 . J_d), G = (ti . tj);
+// the neighbour's coefficients use its own J_d with the same t1, t2, n and G.
+// The penalty length h = 1/|c_d| = |J_d . n| needs no division.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -67,6 +83,25 @@ __device__ __forceinline__ void coef_jacobian(const T (*A)[3], const T *xi, T (*
     }
 }
 
+// 1/a: the hardware reciprocal in FP32 (1 ulp), an exact division in FP64
+template<typename T>
+__device__ __forceinline__ T recip(T a)
+{
+  if constexpr(std::is_same<T, float>::value)
+    return __builtin_amdgcn_rcpf(a);
+  else
+    return T(1) / a;
+}
+
+template<typename T>
+__device__ __forceinline__ T root(T a)
+{
+  if constexpr(std::is_same<T, float>::value)
+    return __builtin_amdgcn_sqrtf(a);
+  else
+    return sqrt(a);
+}
+
 // Ji = J^{-1}, returns det J
 template<int DIM, typename T>
 __device__ __forceinline__ T invert(const T (*J)[3], T (*Ji)[3])
@@ -74,7 +109,7 @@ __device__ __forceinline__ T invert(const T (*J)[3], T (*Ji)[3])
   if constexpr(DIM == 2)
   {
     const T det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const T id = T(1) / det;
+    const T id = recip(det);
     Ji[0][0] = J[1][1] * id;
     Ji[0][1] = -J[0][1] * id;
     Ji[1][0] = -J[1][0] * id;
@@ -87,7 +122,7 @@ __device__ __forceinline__ T invert(const T (*J)[3], T (*Ji)[3])
     const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
     const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
     const T det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    const T id = T(1) / det;
+    const T id = recip(det);
     Ji[0][0] = c00 * id;
     Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
     Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
@@ -113,27 +148,38 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
   constexpr int NFACE = 2 * DIM;
   constexpr int NCORN = 1 << DIM;
   constexpr int NT = NDC <= 64 ? 64 : 128;  // block size
-  constexpr int ST[3] = {1, NP, NP * NP};
+  // lexicographic stride of direction e (by selection: a constant array indexed at
+  // run time would be read from memory)
+  auto stride = [](int e) { return e == 0 ? 1 : (e == 1 ? NP : NP * NP); };
   constexpr int NP2 = DIM == 3 ? NP : 1;    // tangential nodes along the second face direction
 
   __shared__ T U[NDC];
   __shared__ T A0[NDC], A1[NDC], A2[NDC];   // x-stage (gradient) / F (flux) / y-stage (integration)
   __shared__ T B0[NDC], B1[NDC], B2[NDC];   // y-stage (gradient) / z-stage (integration)
   __shared__ T PA[NFACE][2][NF], PB[NFACE][2][NF];  // normal traces; later the projections
-  __shared__ T C[NFACE + 1][NCORN][3];      // map coefficients: own cell [0], neighbours [1 + f]
-  __shared__ T HP[NFACE][2];
+  constexpr int NHC = NCORN / 2;            // corners per face
+  constexpr int NFV = DIM == 3 ? 11 : 5;    // face polynomials per face
+  __shared__ T C[NCORN][3];                 // map coefficients of the cell
+  // per face: tangents (3D: t1 = [0] + q2 [2], t2 = [1] + q1 [2]; 2D: t1 = [0]) and
+  // the normal-direction Jacobian columns of the cell (3D [3..6], 2D [1..2]) and of
+  // the neighbour (3D [7..10], 2D [3..4]) as bilinear (2D: linear) polynomials in
+  // the face coordinates (q1, q2)
+  __shared__ T FG[NFACE][NFV][3];
+  __shared__ T HI[NFACE][2];                // 1/h of the cell and of the neighbour
   __shared__ GeneralTab<NP, T> tl;
   // storage shared by arrays with disjoint lifetimes: the face neighbours (P0-P2)
   // with the face fluxes (P4-P5); the corners (P0-P1) with the tangential traces (P3-P4)
   constexpr int NUN = NFACE * NDC, NFLUX = NFACE * NF * (2 + DIM);
-  constexpr int NTR = 3 * NFACE * 2 * NF, NXD = (NFACE + 1) * NCORN * 3;
+  constexpr int NTR = 3 * NFACE * 2 * NF, NXD = (NCORN + NFACE * NHC) * 3;
   __shared__ T RU[NUN > NFLUX ? NUN : NFLUX];
   __shared__ double RX[(NTR * int(sizeof(T)) + 7) / 8 > NXD ? (NTR * int(sizeof(T)) + 7) / 8 : NXD];
   T(*UN)[NDC] = reinterpret_cast<T(*)[NDC]>(RU);
   T(*JV)[NF] = reinterpret_cast<T(*)[NF]>(RU);
   T(*AV)[NF] = reinterpret_cast<T(*)[NF]>(RU + NFACE * NF);
   T(*FH)[DIM][NF] = reinterpret_cast<T(*)[DIM][NF]>(RU + 2 * NFACE * NF);
-  double(*X)[NCORN][3] = reinterpret_cast<double(*)[NCORN][3]>(RX);
+  // X[c]: corner c of the cell (c < NCORN), X[NCORN + f * NHC + c']: the corner of
+  // the neighbour across face f opposite face corner c'
+  double(*X)[3] = reinterpret_cast<double(*)[3]>(RX);
   T(*VA)[2][NF] = reinterpret_cast<T(*)[2][NF]>(RX);
   T(*DA)[2][NF] = reinterpret_cast<T(*)[2][NF]>(reinterpret_cast<T *>(RX) + NFACE * 2 * NF);
   T(*VB)[2][NF] = reinterpret_cast<T(*)[2][NF]>(reinterpret_cast<T *>(RX) + 2 * NFACE * 2 * NF);
@@ -181,23 +227,31 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
         UN[f][t] = x[(f % 2 == 1 ? cell + st : cell - st) * NDC + t];
       }
   }
-  for(int it = t; it < (NFACE + 1) * NCORN * DIM; it += NT)
   {
-    const int k = it / (NCORN * DIM), c = (it / DIM) % NCORN, a = it % DIM;
-    if(k > 0 && !interior(k - 1)) continue;
-    int cn[3] = {cc[0], cc[1], cc[2]};
-    if(k > 0)
+    // vertex indices fit in 32 bits: (n + 1)^3 < 2^31 for n <= 1000
+    int vs[3] = {1, n + 1, (n + 1) * (n + 1)};
+    const int vb = cc[0] + cc[1] * vs[1] + (DIM == 3 ? cc[2] * vs[2] : 0);
+    for(int it = t; it < (NCORN + NFACE * NHC) * DIM; it += NT)
     {
-      const int f = k - 1;
-      cn[f / 2] += f % 2 == 1 ? 1 : -1;
+      const int c = it / DIM, a = it % DIM;
+      int idx = vb;
+      if(c < NCORN)
+      {
+#pragma unroll
+        for(int e = 0; e < DIM; ++e) idx += ((c >> e) & 1) * vs[e];
+      }
+      else
+      {
+        const int f = (c - NCORN) / NHC, cp = (c - NCORN) % NHC;
+        if(!interior(f)) continue;
+        const int d = f / 2, e1 = d == 0 ? 1 : 0;
+        // offsets: 2 or -1 along d, the face corner's bits along e1 < e2
+#pragma unroll
+        for(int e = 0; e < DIM; ++e)
+          idx += (e == d ? (f % 2 == 1 ? 2 : -1) : (e == e1 ? (cp & 1) : (cp >> 1))) * vs[e];
+      }
+      X[c][a] = vert[int64_t(idx) * DIM + a];
     }
-    int64_t idx = 0, st = 1;
-    for(int e = 0; e < DIM; ++e)
-    {
-      idx += int64_t(cn[e] + ((c >> e) & 1)) * st;
-      st *= (n + 1);
-    }
-    X[k][c][a] = vert[idx * DIM + a];
   }
   __syncthreads();
 
@@ -216,15 +270,57 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
     A0[t] = v;
     A1[t] = g;
   }
-  for(int it = t; it < (NFACE + 1) * NCORN * DIM; it += NT)
+  for(int it = t; it < (NFACE + NCORN) * DIM; it += NT)
   {
-    // A_S = sum over subsets R of S of (-1)^{|S \ R|} X_R
-    const int k = it / (NCORN * DIM), S = (it / DIM) % NCORN, a = it % DIM;
-    if(k > 0 && !interior(k - 1)) continue;
-    double s = 0.0;
-    for(int R = 0; R < NCORN; ++R)
-      if((R & S) == R) s += (__popc(unsigned(S ^ R)) & 1) ? -X[k][R][a] : X[k][R][a];
-    C[k][S][a] = T(s);
+    if(it >= NFACE * DIM)
+    {
+      // A_S = sum over subsets R of S of (-1)^{|S \ R|} X_R
+      const int S = it / DIM - NFACE, a = it % DIM;
+      double s = 0.0;
+      for(int R = 0; R < NCORN; ++R)
+        if((R & S) == R) s += (__popc(unsigned(S ^ R)) & 1) ? -X[R][a] : X[R][a];
+      C[S][a] = T(s);
+      continue;
+    }
+    // face f, coordinate a: Y = the face's corners, E / En = the edges along the
+    // normal direction d of the cell / of the neighbour at those corners (pointing
+    // to increasing x_d), all formed in FP64
+    const int f = it / DIM, a = it % DIM;
+    const int d = f / 2, side = f % 2, e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    const bool inner = interior(f);
+    double Y[NHC], E[NHC], En[NHC];
+#pragma unroll
+    for(int cp = 0; cp < NHC; ++cp)
+    {
+      const int cy = (side << d) | ((cp & 1) << e1) | (DIM == 3 ? ((cp >> 1) << e2) : 0);
+      Y[cp] = X[cy][a];
+      const double z = X[cy ^ (1 << d)][a];
+      E[cp] = side ? Y[cp] - z : z - Y[cp];
+      const double w = inner ? X[NCORN + f * NHC + cp][a] : Y[cp];
+      En[cp] = side ? w - Y[cp] : Y[cp] - w;
+    }
+    if constexpr(DIM == 3)
+    {
+      FG[f][0][a] = T(Y[1] - Y[0]);
+      FG[f][1][a] = T(Y[2] - Y[0]);
+      FG[f][2][a] = T(Y[3] - Y[2] - Y[1] + Y[0]);
+      FG[f][3][a] = T(E[0]);
+      FG[f][4][a] = T(E[1] - E[0]);
+      FG[f][5][a] = T(E[2] - E[0]);
+      FG[f][6][a] = T(E[3] - E[2] - E[1] + E[0]);
+      FG[f][7][a] = T(En[0]);
+      FG[f][8][a] = T(En[1] - En[0]);
+      FG[f][9][a] = T(En[2] - En[0]);
+      FG[f][10][a] = T(En[3] - En[2] - En[1] + En[0]);
+    }
+    else
+    {
+      FG[f][0][a] = T(Y[1] - Y[0]);
+      FG[f][1][a] = T(E[0]);
+      FG[f][2][a] = T(E[1] - E[0]);
+      FG[f][3][a] = T(En[0]);
+      FG[f][4][a] = T(En[1] - En[0]);
+    }
   }
   __syncthreads();
 
@@ -262,14 +358,14 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
     const int d = f / 2, side = f % 2;
     const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
     const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-    const int base = i1 * ST[e1] + (DIM == 3 ? i2 * ST[e2] : 0);
+    const int base = i1 * stride(e1) + (DIM == 3 ? i2 * stride(e2) : 0);
     const T *src = sd == 0 ? U : UN[f];
     const int fs = sd == 0 ? side : 1 - side;
     T a = 0, bs = 0;
 #pragma unroll
     for(int id = 0; id < NP; ++id)
     {
-      const T u = src[base + id * ST[d]];
+      const T u = src[base + id * stride(d)];
       a += tl.f[fs][id] * u;
       bs += tl.fd[fs][id] * u;
     }
@@ -296,7 +392,7 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
     }
     T xi[3] = {tl.qx[ix], tl.qx[iy], DIM == 3 ? tl.qx[iz] : T(0)};
     T J[3][3], Ji[3][3];
-    coef_jacobian<DIM>(C[0], xi, J);
+    coef_jacobian<DIM>(C, xi, J);
     const T det = invert<DIM>(J, Ji);
     const T jxw = fabs(det) * tl.qw[ix] * tl.qw[iy] * (DIM == 3 ? tl.qw[iz] : T(1));
     const T g[3] = {g0, g1, g2};
@@ -369,8 +465,7 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
   for(int pt = t; pt < NFACE * NF; pt += NT)
   {
     const int f = pt / NF, l = pt % NF;
-    const int d = f / 2, side = f % 2;
-    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    const int side = f % 2;
     const bool inner = interior(f);
     const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
     // traces: value, normal and tangential reference derivatives of both sides
@@ -399,57 +494,96 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
       tr[sd][2] = gt1;
       tr[sd][3] = gt2;
     }
-    T xi[3];
-    const T q1 = tl.qx[l1], q2 = tl.qx[l2];
-#pragma unroll
-    for(int e = 0; e < DIM; ++e) xi[e] = e == d ? T(side) : (e == e1 ? q1 : q2);
-    T J[3][3], Jio[3][3];
-    coef_jacobian<DIM>(C[0], xi, J);
-    const T deto = invert<DIM>(J, Jio);
-    // outward unit normal of this cell and surface element (Nanson)
-    T nv[3] = {0, 0, 0}, nn = 0;
+    const T q1 = tl.qx[l1], q2 = DIM == 3 ? tl.qx[l2] : T(0);
+    // tangents and the normal-direction Jacobian columns of both cells
+    T t1[3], t2[3] = {0, 0, 0}, jd[3], jn[3], w[3];
 #pragma unroll
     for(int a = 0; a < DIM; ++a)
     {
-      nv[a] = side == 0 ? -Jio[d][a] : Jio[d][a];
-      nn += nv[a] * nv[a];
+      if constexpr(DIM == 3)
+      {
+        t1[a] = FG[f][0][a] + q2 * FG[f][2][a];
+        t2[a] = FG[f][1][a] + q1 * FG[f][2][a];
+        jd[a] = FG[f][3][a] + q1 * FG[f][4][a] + q2 * (FG[f][5][a] + q1 * FG[f][6][a]);
+        jn[a] = FG[f][7][a] + q1 * FG[f][8][a] + q2 * (FG[f][9][a] + q1 * FG[f][10][a]);
+      }
+      else
+      {
+        t1[a] = FG[f][0][a];
+        jd[a] = FG[f][1][a] + q1 * FG[f][2][a];
+        jn[a] = FG[f][3][a] + q1 * FG[f][4][a];
+      }
     }
-    nn = sqrt(nn);
-    const T inn = T(1) / nn;
-#pragma unroll
-    for(int a = 0; a < DIM; ++a) nv[a] *= inn;
-    const T wq = tl.qw[l1] * (DIM == 3 ? tl.qw[l2] : T(1));
-    const T dS = fabs(deto) * nn * wq;
-    // reference-gradient coefficients of the normal derivative: co = J^{-1} n
-    T co[3] = {0, 0, 0};
-#pragma unroll
-    for(int bb = 0; bb < DIM; ++bb)
+    if constexpr(DIM == 3)
     {
-      T s = 0;
-#pragma unroll
-      for(int a = 0; a < DIM; ++a) s += Jio[bb][a] * nv[a];
-      co[bb] = s;
+      w[0] = t1[1] * t2[2] - t1[2] * t2[1];
+      w[1] = t1[2] * t2[0] - t1[0] * t2[2];
+      w[2] = t1[0] * t2[1] - t1[1] * t2[0];
     }
-    const T dno = tr[0][1] * co[d] + tr[0][2] * co[e1] + (DIM == 3 ? tr[0][3] * co[e2] : T(0));
+    else
+    {
+      w[0] = t1[1];
+      w[1] = -t1[0];
+    }
+    T ww = 0, wj = 0;
+#pragma unroll
+    for(int a = 0; a < DIM; ++a)
+    {
+      ww += w[a] * w[a];
+      wj += w[a] * jd[a];
+    }
+    const T nw = root(ww), inw = recip(nw);
+    // outward: n . J_d > 0 on the face x_d = 1 of the cell, < 0 on x_d = 0
+    const T sg = ((wj > T(0)) == (side == 1)) ? inw : -inw;
+    T nv[3];
+#pragma unroll
+    for(int a = 0; a < DIM; ++a) nv[a] = sg * w[a];
+    const T wq = tl.qw[l1] * (DIM == 3 ? tl.qw[l2] : T(1));
+    const T dS = nw * wq;
+    // Gram matrix of the tangents (2D: t1 . t1) and its inverse
+    T g11 = 0, g12 = 0, g22 = 0;
+#pragma unroll
+    for(int a = 0; a < DIM; ++a)
+    {
+      g11 += t1[a] * t1[a];
+      g12 += t1[a] * t2[a];
+      g22 += t2[a] * t2[a];
+    }
+    const T iG = DIM == 3 ? recip(g11 * g22 - g12 * g12) : recip(g11);
+    // c = J^{-1} n in the face frame: [0] normal direction, [1] e1, [2] e2
+    auto coeffs = [&](const T *jc, T *c) {
+      T jnv = 0, r1 = 0, r2 = 0;
+#pragma unroll
+      for(int a = 0; a < DIM; ++a)
+      {
+        jnv += jc[a] * nv[a];
+        r1 += t1[a] * jc[a];
+        r2 += t2[a] * jc[a];
+      }
+      const T cd = recip(jnv);
+      c[0] = cd;
+      if constexpr(DIM == 3)
+      {
+        c[1] = -cd * (g22 * r1 - g12 * r2) * iG;
+        c[2] = -cd * (g11 * r2 - g12 * r1) * iG;
+      }
+      else
+      {
+        c[1] = -cd * r1 * iG;
+        c[2] = T(0);
+      }
+    };
+    T co[3];
+    coeffs(jd, co);
+    const T dno = tr[0][1] * co[0] + tr[0][2] * co[1] + (DIM == 3 ? tr[0][3] * co[2] : T(0));
     T jv, av, fnd, cnd = T(1);
     if(inner)
     {
-      xi[d] = T(1 - side);
-      T Jn[3][3], Jin[3][3];
-      coef_jacobian<DIM>(C[1 + f], xi, Jn);
-      invert<DIM>(Jn, Jin);
-      T cn[3] = {0, 0, 0};
-#pragma unroll
-      for(int bb = 0; bb < DIM; ++bb)
-      {
-        T s = 0;
-#pragma unroll
-        for(int a = 0; a < DIM; ++a) s += Jin[bb][a] * nv[a];
-        cn[bb] = s;
-      }
-      cnd = cn[d];
+      T cn[3];
+      coeffs(jn, cn);
+      cnd = cn[0];
       const T dnn =
-        tr[1][1] * cn[d] + tr[1][2] * cn[e1] + (DIM == 3 ? tr[1][3] * cn[e2] : T(0));
+        tr[1][1] * cn[0] + tr[1][2] * cn[1] + (DIM == 3 ? tr[1][3] * cn[2] : T(0));
       const T jump = tr[0][0] - tr[1][0];
       jv = dS * jump;
       av = -dS * T(0.5) * (dno + dnn);
@@ -464,13 +598,13 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
     JV[f][l] = jv;
     AV[f][l] = av;
 #pragma unroll
-    for(int bb = 0; bb < DIM; ++bb) FH[f][bb][l] = fnd * co[bb];
-    // penalty lengths h = 1/|(J^{-1} n)_d| at the first face quadrature point of
-    // both cells (apps/laplace_integrator.h:651-656, 692-693)
+    for(int bb = 0; bb < DIM; ++bb) FH[f][bb][l] = fnd * co[bb];  // face frame
+    // inverse penalty lengths 1/h = |(J^{-1} n)_d| at the first face quadrature
+    // point of both cells (apps/laplace_integrator.h:651-656, 692-693)
     if(l == 0)
     {
-      HP[f][0] = T(1) / fabs(co[d]);
-      HP[f][1] = inner ? T(1) / fabs(cnd) : T(1);
+      HI[f][0] = fabs(co[0]);
+      HI[f][1] = inner ? fabs(cnd) : T(1);
     }
   }
   __syncthreads();
@@ -507,29 +641,73 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
       U[t] = s1;
     }
   }
-  for(int it = t; it < NFACE * NF; it += NT)
+  if constexpr(DIM == 3)
   {
-    const int f = it / NF, m = it % NF;
-    const int d = f / 2;
-    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
-    const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-    const T sigma = interior(f) ? T(ip) * gamma * (T(0.5) / HP[f][0] + T(0.5) / HP[f][1])
-                                : T(ip) * T(2) * gamma / HP[f][0];
-    T pv = 0, pd = 0;
-#pragma unroll
-    for(int l = 0; l < NF; ++l)
+    // projections sum-factorised over the two face coordinates, first over l1
+    // (into the trace storage, last read in P4): at (i1, l2)
+    //   SA = sum_l1 phi_i1(l1) (sigma JV + AV) + phi'_i1(l1) FH_e1
+    //   SB = sum_l1 phi_i1(l1) FH_e2,  SC = sum_l1 phi_i1(l1) FH_n
+    T(*SA)[NF] = reinterpret_cast<T(*)[NF]>(RX);
+    T(*SB)[NF] = SA + NFACE;
+    T(*SC)[NF] = SA + 2 * NFACE;
+    for(int it = t; it < NFACE * NF; it += NT)
     {
-      const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
-      const T t1v = tl.V[l1 * NP + i1], t1d = tl.Dv[l1 * NP + i1];
-      const T t2v = DIM == 3 ? tl.V[l2 * NP + i2] : T(1);
-      const T t2d = DIM == 3 ? tl.Dv[l2 * NP + i2] : T(0);
-      const T wvv = t1v * t2v;
-      pv += wvv * (sigma * JV[f][l] + AV[f][l]) + t1d * t2v * FH[f][e1][l];
-      if(DIM == 3) pv += t1v * t2d * FH[f][e2][l];
-      pd += wvv * FH[f][d][l];
+      const int f = it / NF, m = it % NF;
+      const int i1 = m % NP, l2 = m / NP;
+      const T sigma = interior(f) ? T(ip) * gamma * T(0.5) * (HI[f][0] + HI[f][1])
+                                  : T(ip) * T(2) * gamma * HI[f][0];
+      T sa = 0, sb = 0, sc = 0;
+#pragma unroll
+      for(int l1 = 0; l1 < NP; ++l1)
+      {
+        const int l = l1 + NP * l2;
+        const T v = tl.V[l1 * NP + i1], dv = tl.Dv[l1 * NP + i1];
+        sa += v * (sigma * JV[f][l] + AV[f][l]) + dv * FH[f][1][l];
+        sb += v * FH[f][2][l];
+        sc += v * FH[f][0][l];
+      }
+      SA[f][m] = sa;
+      SB[f][m] = sb;
+      SC[f][m] = sc;
     }
-    PA[f][0][m] = pv;
-    PB[f][0][m] = pd;
+    __syncthreads();
+    // then over l2: the projections on phi_i1 phi_i2 (value) and on the normal
+    // derivative's coefficient
+    for(int it = t; it < NFACE * NF; it += NT)
+    {
+      const int f = it / NF, m = it % NF;
+      const int i1 = m % NP, i2 = m / NP;
+      T pv = 0, pd = 0;
+#pragma unroll
+      for(int l2 = 0; l2 < NP; ++l2)
+      {
+        const int r = i1 + NP * l2;
+        const T v = tl.V[l2 * NP + i2], dv = tl.Dv[l2 * NP + i2];
+        pv += v * SA[f][r] + dv * SB[f][r];
+        pd += v * SC[f][r];
+      }
+      PA[f][0][m] = pv;
+      PB[f][0][m] = pd;
+    }
+  }
+  else
+  {
+    for(int it = t; it < NFACE * NF; it += NT)
+    {
+      const int f = it / NF, i1 = it % NF;
+      const T sigma = interior(f) ? T(ip) * gamma * T(0.5) * (HI[f][0] + HI[f][1])
+                                  : T(ip) * T(2) * gamma * HI[f][0];
+      T pv = 0, pd = 0;
+#pragma unroll
+      for(int l = 0; l < NF; ++l)
+      {
+        const T v = tl.V[l * NP + i1];
+        pv += v * (sigma * JV[f][l] + AV[f][l]) + tl.Dv[l * NP + i1] * FH[f][1][l];
+        pd += v * FH[f][0][l];
+      }
+      PA[f][0][i1] = pv;
+      PB[f][0][i1] = pd;
+    }
   }
   __syncthreads();
 
