@@ -591,15 +591,25 @@ __global__ void k_patch_dof_indices(const uint32_t *__restrict__ vtx, const int 
   out[g] = uint32_t(patch_cell<DIM, NP>(v, n, i0, i1, i2) * NDC + patch_local<DIM, NP>(i0, i1, i2));
 }
 
-// s * w / lam: correctly rounded division in FP64; in FP32 the hardware
-// reciprocal (1 ulp), which is well inside the FP32 patch solve's rounding
+// s * w / lam.  FP32: the hardware reciprocal (1 ulp), well inside the FP32 patch
+// solve's rounding.  FP64: the hardware reciprocal refined by two Newton steps
+// r <- r + r (1 - lam r) (each doubles the correct bits; within an ulp or two of
+// 1/lam), five instructions instead of the correctly rounded division's scale,
+// refinement and fix-up sequence
 template<typename CT>
 __device__ __forceinline__ CT eig_scale(CT s, CT w, CT lam)
 {
   if constexpr(std::is_same<CT, float>::value)
     return s * w * __builtin_amdgcn_rcpf(lam);
   else
-    return s * w / lam;
+  {
+    double r = __builtin_amdgcn_rcp(lam);
+    double e = fma(-lam, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-lam, r, 1.0);
+    r = fma(r, e, r);
+    return s * w * r;
+  }
 }
 
 // o = S^T r (forward) and r = S o (backward) along one line, S[a * K + i] =

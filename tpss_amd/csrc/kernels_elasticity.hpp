@@ -19,7 +19,9 @@
 //     sides contracted along the normal (the neighbour's straight from global
 //     memory; the face neighbours' cell indices are formed once per block),
 //     fluxes per face point, fluxes projected on the tangential test
-//     functions, and every thread adds the terms of its test function.
+//     functions (3D: sum-factorised, one tangential direction after the other,
+//     through the cell-stage arrays, which the faces leave free), and every
+//     thread adds the terms of its test function.
 // Faces are evaluated element-centrically (every output entry is written by
 // exactly one thread).
 #pragma once
@@ -314,43 +316,46 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
     constexpr int E2 = DIM == 3 ? (D == 2 ? 1 : 2) : E1;
     __syncthreads();  // PA / PB of the previous direction have been read
     // traces along the normal: value (PA) and normal derivative (PB) of both
-    // cell sides for every component and tangential node
-    for(int o = t; o < CPB * 2 * 2 * DIM * NF; o += NT)
-    {
-      const int s = o / (2 * 2 * DIM * NF), r = o % (2 * 2 * DIM * NF);
-      const int side = r / (2 * DIM * NF), sd = (r / (DIM * NF)) % 2;
-      const int c = (r / NF) % DIM, m = r % NF;
-      if(!has_cell(s)) continue;
-      const bool inner = nbc[s][2 * D + side] >= 0;
-      if(sd == 1 && !inner) continue;
-      const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-      const int base = i1 * ST[E1] + (DIM == 3 ? i2 * ST[E2] : 0);
-      const int fs = sd == 0 ? side : 1 - side;
-      T a = 0, bs = 0;
-      if(sd == 0)
-      {
+    // cell sides for every component and tangential node; the cell's own side
+    // (from LDS) and the neighbour's (from global memory) in separate passes, so
+    // that a wave runs one of the two branches per iteration
 #pragma unroll
-        for(int id = 0; id < NP; ++id)
-        {
-          const T u = U[s][c][base + id * ST[D]];
-          a += tl.f[fs][id] * u;
-          bs += tl.fd[fs][id] * u;
-        }
-      }
-      else
+    for(int sd = 0; sd < 2; ++sd)
+      for(int o = t; o < CPB * 2 * DIM * NF; o += NT)
       {
-        const VT *src = x + c * Ns + nbc[s][2 * D + side] * NDC + base;
-#pragma unroll
-        for(int id = 0; id < NP; ++id)
+        const int s = o / (2 * DIM * NF), side = (o / (DIM * NF)) % 2;
+        const int c = (o / NF) % DIM, m = o % NF;
+        if(!has_cell(s)) continue;
+        const bool inner = nbc[s][2 * D + side] >= 0;
+        if(sd == 1 && !inner) continue;
+        const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+        const int base = i1 * ST[E1] + (DIM == 3 ? i2 * ST[E2] : 0);
+        const int fs = sd == 0 ? side : 1 - side;
+        T a = 0, bs = 0;
+        if(sd == 0)
         {
-          const T u = src[id * ST[D]];
-          a += tl.f[fs][id] * u;
-          bs += tl.fd[fs][id] * u;
+#pragma unroll
+          for(int id = 0; id < NP; ++id)
+          {
+            const T u = U[s][c][base + id * ST[D]];
+            a += tl.f[fs][id] * u;
+            bs += tl.fd[fs][id] * u;
+          }
         }
+        else
+        {
+          const VT *src = x + c * Ns + nbc[s][2 * D + side] * NDC + base;
+#pragma unroll
+          for(int id = 0; id < NP; ++id)
+          {
+            const T u = src[id * ST[D]];
+            a += tl.f[fs][id] * u;
+            bs += tl.fd[fs][id] * u;
+          }
+        }
+        PA[s][side][sd][c][m] = a;
+        PB[s][side][sd][c][m] = bs;
       }
-      PA[s][side][sd][c][m] = a;
-      PB[s][side][sd][c][m] = bs;
-    }
     __syncthreads();
     // fluxes at every face point
     for(int o = t; o < CPB * 2 * NF; o += NT)
@@ -437,27 +442,73 @@ k_elasticity(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
     }
     __syncthreads();
     // projections on the tangential test functions (into PA / PB [.][0])
-    for(int o = t; o < CPB * 2 * DIM * NF; o += NT)
+    if constexpr(DIM == 3)
     {
-      const int s = o / (2 * DIM * NF), side = (o / (DIM * NF)) % 2;
-      const int c = (o / NF) % DIM, m = o % NF;
-      if(!has_cell(s)) continue;
-      const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-      T pv = 0, pd = 0;
-#pragma unroll
-      for(int l = 0; l < NF; ++l)
+      // sum-factorised: along e1 into the cell-stage arrays (free during the faces),
+      // SA = sum_l1 phi_i1 FA + phi'_i1 FB_E1, SB = sum_l1 phi_i1 FB_E2,
+      // SC = sum_l1 phi_i1 FB_D at (i1, l2); then along e2
+      constexpr int NQ = CPB * 2 * DIM * NF;
+      static_assert(3 * NQ <= CPB * NS * NDC, "projection stage fits the stage arrays");
+      T *SA = &R2[0][0][0], *SB = SA + NQ, *SC = SA + 2 * NQ;
+      for(int o = t; o < NQ; o += NT)
       {
-        const int l1 = l % NP, l2 = DIM == 3 ? l / NP : 0;
-        const T t1v = tl.V[l1 * NP + i1], t1d = tl.G[l1 * NP + i1];
-        const T t2v = DIM == 3 ? tl.V[l2 * NP + i2] : T(1);
-        const T t2d = DIM == 3 ? tl.G[l2 * NP + i2] : T(0);
-        const T wvv = t1v * t2v;
-        pv += wvv * FA[s][side][c][l] + t1d * t2v * FB[s][side][c][E1][l];
-        if(DIM == 3) pv += t1v * t2d * FB[s][side][c][E2][l];
-        pd += wvv * FB[s][side][c][D][l];
+        const int s = o / (2 * DIM * NF), side = (o / (DIM * NF)) % 2;
+        const int c = (o / NF) % DIM, m = o % NF;
+        if(!has_cell(s)) continue;
+        const int i1 = m % NP, l2 = m / NP;
+        T sa = 0, sb = 0, sc = 0;
+#pragma unroll
+        for(int l1 = 0; l1 < NP; ++l1)
+        {
+          const int l = l1 + NP * l2;
+          const T v = tl.V[l1 * NP + i1], g = tl.G[l1 * NP + i1];
+          sa += v * FA[s][side][c][l] + g * FB[s][side][c][E1][l];
+          sb += v * FB[s][side][c][E2][l];
+          sc += v * FB[s][side][c][D][l];
+        }
+        SA[o] = sa;
+        SB[o] = sb;
+        SC[o] = sc;
       }
-      PA[s][side][0][c][m] = pv;
-      PB[s][side][0][c][m] = pd;
+      __syncthreads();
+      for(int o = t; o < NQ; o += NT)
+      {
+        const int s = o / (2 * DIM * NF), side = (o / (DIM * NF)) % 2;
+        const int c = (o / NF) % DIM, m = o % NF;
+        if(!has_cell(s)) continue;
+        const int i1 = m % NP, i2 = m / NP, base = o - m;
+        T pv = 0, pd = 0;
+#pragma unroll
+        for(int l2 = 0; l2 < NP; ++l2)
+        {
+          const int r = base + i1 + NP * l2;
+          const T v = tl.V[l2 * NP + i2], g = tl.G[l2 * NP + i2];
+          pv += v * SA[r] + g * SB[r];
+          pd += v * SC[r];
+        }
+        PA[s][side][0][c][m] = pv;
+        PB[s][side][0][c][m] = pd;
+      }
+    }
+    else
+    {
+      // 2D: one tangential direction (NF = NP)
+      for(int o = t; o < CPB * 2 * DIM * NF; o += NT)
+      {
+        const int s = o / (2 * DIM * NF), side = (o / (DIM * NF)) % 2;
+        const int c = (o / NF) % DIM, i1 = o % NF;
+        if(!has_cell(s)) continue;
+        T pv = 0, pd = 0;
+#pragma unroll
+        for(int l = 0; l < NF; ++l)
+        {
+          const T v = tl.V[l * NP + i1];
+          pv += v * FA[s][side][c][l] + tl.G[l * NP + i1] * FB[s][side][c][E1][l];
+          pd += v * FB[s][side][c][D][l];
+        }
+        PA[s][side][0][c][i1] = pv;
+        PB[s][side][0][c][i1] = pd;
+      }
     }
     __syncthreads();
     if(act)
