@@ -103,6 +103,47 @@ def test_nonfinite_output_is_reported():
     assert e.value.code == t.E_NONFINITE
 
 
+@pytest.mark.parametrize("dim,p,n", [(3, 3, 6), (3, 2, 4), (3, 3, 16)])
+def test_nonfinite_check_of_the_residual_free_passes(dim, p, n):
+    """The residual-free color passes check the values they store themselves
+    (no separate pass over the output); the additive path and the accumulating
+    entry points keep the pass over the output.  (3, 3, 16): colors of at most
+    512 patches run the four-wave pass."""
+    t = _tpss()
+    lv, sm, ref = make(dim, p, n, t.MULTIPLICATIVE)
+    N = ref.N
+    src = o.splitmix_uniform(3, N)
+    # a NaN in the right-hand side of a Richardson step (read by the patches
+    # holding that DoF) reaches the output
+    b = src.copy()
+    b[N // 2] = np.nan
+    sm.step(dev(o.splitmix_uniform(4, N)), dev(b))
+    with pytest.raises(t.TPSSError) as e:
+        sm.finite_status()
+    assert e.value.code == t.E_NONFINITE
+    # Tvmult with an Inf in the source
+    bad = src.copy()
+    bad[N - 1] = np.inf
+    dst = dev(np.zeros(N))
+    sm.Tvmult(dst, dev(bad))
+    with pytest.raises(t.TPSSError) as e:
+        sm.finite_status()
+    assert e.value.code == t.E_NONFINITE
+    # vmult_add: the passes write a temporary; a NaN already in dst is found by
+    # the pass over the output
+    d = np.zeros(N)
+    d[7] = np.nan
+    sm.vmult_add(dev(d), dev(src))
+    with pytest.raises(t.TPSSError) as e:
+        sm.finite_status()
+    assert e.value.code == t.E_NONFINITE
+    # finite data: nothing reported, and the results are unchanged
+    dst = dev(np.zeros(N))
+    sm.precond_vmult(dst, dev(src))
+    sm.finite_status()
+    assert rel_err(host(dst), ref.precond_vmult(src)) < 1e-12
+
+
 def test_gmg_solve_reports_nonfinite_and_no_convergence():
     t = _tpss()
     g = t.GMGSolver(2, 2, 2, 3)

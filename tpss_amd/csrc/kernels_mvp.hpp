@@ -125,7 +125,8 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
                   const CT *__restrict__ S_tab, const CT *__restrict__ lam_tab,
                   const uint32_t *__restrict__ eo_tab, const float *__restrict__ omega,
                   const CT relaxation, const int patch_begin,
-                  const int patch_end, const int n, const int x_is_zero)
+                  const int patch_end, const int n, const int x_is_zero,
+                  int *__restrict__ nonfinite)
 {
   using PS = PatchShape<DIM, K>;
   using MS = MvpShape<DIM, K, FD>;
@@ -426,6 +427,14 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
     load_row<NP>(x + (c_lo + 1) * NDC + lyz, old + NP);
 #pragma unroll
     for(int a = 0; a < K; ++a) o[a] += (CT(1) - w) * old[a];
+  }
+  if(nonfinite)
+  {
+    // the non-finite check of the caller's output, on the values stored here
+    bool bad = false;
+#pragma unroll
+    for(int a = 0; a < K; ++a) bad = bad || !__builtin_isfinite(o[a]);
+    if(bad) *nonfinite = 1;
   }
   store_row<NP>(x + c_lo * NDC + lyz, o);
   store_row<NP>(x + (c_lo + 1) * NDC + lyz, o + NP);

@@ -69,7 +69,7 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
                   const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
                   const CT *__restrict__ S_tab, const CT *__restrict__ lam_tab,
                   const float *__restrict__ omega, const CT relaxation, const int patch_begin,
-                  const int n, const int x_is_zero)
+                  const int n, const int x_is_zero, int *__restrict__ nonfinite)
 {
   constexpr int K = 8, NP = 4, NDC = 64;
   constexpr int SY = 8, SZ = 72;     // solve buffer (x, y, z) -> x + SY y + SZ z
@@ -304,6 +304,7 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
 #pragma unroll
     for(int k = 0; k < 2; ++k) o[k] += (CT(1) - w) * old[k];
   }
+  if(nonfinite && !(__builtin_isfinite(o[0]) && __builtin_isfinite(o[1]))) *nonfinite = 1;
   px[0] = VT(o[0]);
   px[1] = VT(o[1]);
 }

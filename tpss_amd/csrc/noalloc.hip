@@ -86,6 +86,7 @@ int tpss_precond_vmult_noalloc(tpss_smoother *sm, void *dst, const void *src, vo
         sm->timing = timing;
       }
     } restore{sm, timing};
+    PassCheck armed(sm);
     precond_vmult_any(sm, dst, src, as_stream(stream), false, 1.0);
     check_finite(sm, dst, as_stream(stream));
   });

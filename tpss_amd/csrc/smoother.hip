@@ -511,7 +511,7 @@ void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, b
       if constexpr(DIM == 3 && K == 8 && std::is_same<CT, float>::value)
         hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT, 1>), dim3(grid), dim3(MS::THREADS), 0,
                            s, b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, nullptr, om, relax,
-                           begin, end, ls.n, x_is_zero ? 1 : 0);
+                           begin, end, ls.n, x_is_zero ? 1 : 0, sm->pass_flag);
       else
         throw Error(TPSS_E_UNSUPPORTED, "BF16-MFMA patch solves are built for 3D Q3 (k = 8)");
     }
@@ -523,13 +523,13 @@ void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, b
           // few patches (coarse levels): one patch over four waves
           hipLaunchKernelGGL((k_patch_mvp_split<VT, CT>), dim3(unsigned(end - begin)), dim3(256),
                              0, s, b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, om, relax,
-                             begin, ls.n, x_is_zero ? 1 : 0);
+                             begin, ls.n, x_is_zero ? 1 : 0, sm->pass_flag);
           TPSS_HIP(hipGetLastError());
           return;
         }
       hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT>), dim3(grid), dim3(MS::THREADS), 0, s,
                          b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, ls.d_eo.ptr, om, relax,
-                         begin, end, ls.n, x_is_zero ? 1 : 0);
+                         begin, end, ls.n, x_is_zero ? 1 : 0, sm->pass_flag);
     }
     TPSS_HIP(hipGetLastError());
   });
@@ -654,7 +654,10 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
       SectionTimer t(sm, 1, s);
       mvp_fused_pass<VT, CT>(sm, src, out, ls.color_ptr[c], ls.color_ptr[c + 1], k == 0, s);
     }
-    if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
+    if(accumulate)
+      axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
+    else if(sm->pass_flag)
+      sm->pass_checked = true;  // every cell belongs to a patch: dst is covered
     return;
   }
   VT *res = (VT *)sm->scratch;
@@ -794,6 +797,7 @@ void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_
       else
         fused_sweep<float>(sm, (float *)dst, (const float *)src, x_is_zero && k == 0, s);
     }
+    if(sm->pass_flag) sm->pass_checked = true;
     return;
   }
   const size_t bytes = size_t(lv->N) * elem_size(lv->vec_prec);
@@ -844,14 +848,21 @@ void check_finite(tpss_smoother *sm, const void *x, hipStream_t s)
   if(!sm->check_finite) return;
   const tpss_level *lv = sm->level;
   sm->nonfinite.ensure();
-  const unsigned grid = unsigned(std::min<int64_t>(2048, (lv->N + 255) / 256));
-  if(lv->vec_prec == TPSS_F64)
-    hipLaunchKernelGGL((k_nonfinite<double>), dim3(grid), dim3(256), 0, s, (const double *)x,
-                       lv->N, sm->nonfinite.dev);
-  else
-    hipLaunchKernelGGL((k_nonfinite<float>), dim3(grid), dim3(256), 0, s, (const float *)x,
-                       lv->N, sm->nonfinite.dev);
-  TPSS_HIP(hipGetLastError());
+  // a residual-free sweep that wrote the whole output has checked every value
+  // it stored (PassCheck); otherwise one pass over the output
+  const bool checked = sm->pass_checked;
+  sm->pass_checked = false;
+  if(!checked)
+  {
+    const unsigned grid = unsigned(std::min<int64_t>(2048, (lv->N + 255) / 256));
+    if(lv->vec_prec == TPSS_F64)
+      hipLaunchKernelGGL((k_nonfinite<double>), dim3(grid), dim3(256), 0, s, (const double *)x,
+                         lv->N, sm->nonfinite.dev);
+    else
+      hipLaunchKernelGGL((k_nonfinite<float>), dim3(grid), dim3(256), 0, s, (const float *)x,
+                         lv->N, sm->nonfinite.dev);
+    TPSS_HIP(hipGetLastError());
+  }
   if(sm->check_finite == 2)
   {
     TPSS_HIP(hipStreamSynchronize(s));
@@ -965,6 +976,7 @@ int tpss_precond_vmult(tpss_smoother *sm, void *dst, const void *src, void *stre
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
     report_nonfinite(sm);
+    PassCheck armed(sm);
     precond_vmult_any(sm, dst, src, as_stream(stream), false, 1.0);
     check_finite(sm, dst, as_stream(stream));
   });
@@ -976,6 +988,7 @@ int tpss_precond_vmult_add(tpss_smoother *sm, void *dst, const void *src, void *
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
     report_nonfinite(sm);
+    PassCheck armed(sm);
     precond_vmult_any(sm, dst, src, as_stream(stream), true, 1.0);
     check_finite(sm, dst, as_stream(stream));
   });
@@ -987,6 +1000,7 @@ int tpss_precond_Tvmult(tpss_smoother *sm, void *dst, const void *src, void *str
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
     report_nonfinite(sm);
+    PassCheck armed(sm);
     precond_vmult_any(sm, dst, src, as_stream(stream), false, 1.0, true);
     check_finite(sm, dst, as_stream(stream));
   });
@@ -998,6 +1012,7 @@ int tpss_precond_Tvmult_add(tpss_smoother *sm, void *dst, const void *src, void 
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
     report_nonfinite(sm);
+    PassCheck armed(sm);
     precond_vmult_any(sm, dst, src, as_stream(stream), true, 1.0, true);
     check_finite(sm, dst, as_stream(stream));
   });
@@ -1009,6 +1024,7 @@ int tpss_smoother_step(tpss_smoother *sm, void *dst, const void *src, void *stre
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
     report_nonfinite(sm);
+    PassCheck armed(sm);
     smoother_step_any(sm, dst, src, as_stream(stream));
     check_finite(sm, dst, as_stream(stream));
   });
@@ -1020,6 +1036,7 @@ int tpss_smoother_vmult(tpss_smoother *sm, void *dst, const void *src, void *str
     TPSS_REQUIRE(sm && dst && src, "null argument");
     TPSS_REQUIRE(dst != src, "dst and src must not alias");
     report_nonfinite(sm);
+    PassCheck armed(sm);
     fill_zero(dst, sm->level->vec_prec, sm->level->N, as_stream(stream));
     smoother_step_any(sm, dst, src, as_stream(stream), true);
     check_finite(sm, dst, as_stream(stream));

@@ -82,6 +82,12 @@ struct tpss_smoother
   // synchronisation in the call itself
   int check_finite = 1;
   tpss::PinnedFlag nonfinite;
+  // set by an entry point for its own call: the residual-free color passes
+  // check the values they store and set this flag (the device address of
+  // `nonfinite`); pass_checked records that they covered the whole output, so
+  // that check_finite needs no separate pass over it
+  int *pass_flag = nullptr;
+  bool pass_checked = false;
   bool fused_mvp = true;  // residual-free color passes where the mesh allows them
   tpss::Timer timers[5];
   struct PendingSection
@@ -104,6 +110,21 @@ void precond_vmult_any(tpss_smoother *sm, void *dst, const void *src, hipStream_
 size_t precond_scratch_bytes(const tpss_smoother *sm);
 void report_nonfinite(tpss_smoother *sm);
 void check_finite(tpss_smoother *sm, const void *x, hipStream_t s);
+
+// arms the in-pass non-finite check of the residual-free color passes for the
+// duration of one entry-point call
+struct PassCheck
+{
+  tpss_smoother *sm;
+  explicit PassCheck(tpss_smoother *s) : sm(s)
+  {
+    sm->pass_checked = false;
+    sm->pass_flag = sm->check_finite ? sm->nonfinite.dev : nullptr;
+  }
+  PassCheck(const PassCheck &) = delete;
+  PassCheck &operator=(const PassCheck &) = delete;
+  ~PassCheck() { sm->pass_flag = nullptr; }
+};
 void smoother_step_any(tpss_smoother *sm, void *dst, const void *src, hipStream_t s,
                        bool x_is_zero = false);
 template<typename VT>
