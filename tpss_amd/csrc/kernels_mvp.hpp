@@ -100,6 +100,54 @@ __device__ __forceinline__ void fd_mfma_pair(float (*B2)[512], const float (*S)[
 // others compute; longer lines (K = 10) hold more values per thread: 3.  FP64
 // solves unconstrained (3D K = 8: 186 VGPRs, 2 waves; forcing 3 spills and
 // measured 20 % slower).
+// value of v in lane SRC of this lane's quad (DPP quad_perm broadcast)
+template<int SRC>
+__device__ __forceinline__ int quad_bcast_i(int v)
+{
+  return __builtin_amdgcn_update_dpp(0, v, SRC * 0x55, 0xF, 0xF, false);
+}
+template<typename CT>
+__device__ __forceinline__ CT quad_bcast_rt(CT v, int src)
+{
+  if constexpr(std::is_same<CT, float>::value)
+  {
+    const int i = __float_as_int(v);
+    const int r = src == 0 ? quad_bcast_i<0>(i)
+                : src == 1 ? quad_bcast_i<1>(i)
+                : src == 2 ? quad_bcast_i<2>(i)
+                           : quad_bcast_i<3>(i);
+    return __int_as_float(r);
+  }
+  else
+  {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    int rl, rh;
+    if(src == 0)
+      rl = quad_bcast_i<0>(lo), rh = quad_bcast_i<0>(hi);
+    else if(src == 1)
+      rl = quad_bcast_i<1>(lo), rh = quad_bcast_i<1>(hi);
+    else if(src == 2)
+      rl = quad_bcast_i<2>(lo), rh = quad_bcast_i<2>(hi);
+    else
+      rl = quad_bcast_i<3>(lo), rh = quad_bcast_i<3>(hi);
+    return __hiloint2double(rh, rl);
+  }
+}
+
+// value of v in lane `lane` of the wave (ds_bpermute)
+template<typename CT>
+__device__ __forceinline__ CT lane_read(CT v, int lane)
+{
+  if constexpr(std::is_same<CT, float>::value)
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v)));
+  else
+  {
+    const int lo = __builtin_amdgcn_ds_bpermute(lane << 2, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(lane << 2, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+  }
+}
+
 template<int K, typename CT, int FD>
 constexpr int mvp_min_waves()
 {
@@ -160,34 +208,57 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   // matrices would need twice the scalar registers and spill.)
   constexpr bool WAVE_PATCH = FD == 0 && L == 64 && MS::THREADS == PPB * 64;
   constexpr bool S_SCALAR = WAVE_PATCH && std::is_same<CT, float>::value;
-  // one wave per patch with the matrices not in scalar registers (FP64): the
-  // even-odd products take the half matrices straight from scalar loads, so
-  // only patches without mirror symmetry stage the full matrices in LDS
-  bool stage_S = true;
-  if constexpr(WAVE_PATCH && !S_SCALAR)
-  {
-    const int jw = min(j0, patch_end - 1);
-    stage_S = !(eo_tab && (eo_tab[fac[size_t(jw) * DIM]] & eo_tab[fac[size_t(jw) * DIM + 1]] &
-                           eo_tab[fac[size_t(jw) * DIM + 2]]) != 0u);
-  }
-  if constexpr(!S_SCALAR)
-    for(int idx = threadIdx.x; stage_S && idx < PPB * DIM * K * K; idx += blockDim.x)
-    {
-      const int q = idx / (DIM * K * K), r = idx % (DIM * K * K);
-      const int d = r / (K * K), e = r % (K * K);
-      const int jj = j0 + q;
-      Ss[q][d][e] = jj < patch_end ? S_tab[size_t(fac[size_t(jj) * DIM + d]) * K * K + e] : CT(0);
-    }
-  for(int idx = threadIdx.x; idx < PPB * DIM * K; idx += blockDim.x)
-  {
-    const int q = idx / (DIM * K), r = idx % (DIM * K);
-    const int d = r / K, e = r % K;
-    const int jj = j0 + q;
-    ls[q][d][e] = jj < patch_end ? lam_tab[size_t(fac[size_t(jj) * DIM + d]) * K + e] : CT(1);
-  }
+  // One patch per wave, 3D, K = 8 (the fine levels): every global load of the
+  // pass is issued in two batches -- the patch's vertex and factor rows, then
+  // its factor tables, b and the outer neighbour rows -- before the first wait,
+  // so that a wave waits for two memory round trips, not for one per table and
+  // neighbour.  Each lane loads one row of the y- and z-neighbour planes and
+  // takes the other three rows of a plane from the lanes that loaded them: the
+  // lanes sharing a y-plane form a quad (DPP), those sharing a z-plane sit 8
+  // lanes apart (ds_bpermute).  The patch tables are staged unconditionally
+  // (the full matrices are unused by the even-odd products).
+  constexpr bool BATCH = WAVE_PATCH && DIM == 3 && K == 8 && !std::is_same<CT, float>::value;
   int v[3] = {1, 1, 1};
-  if(valid) unpack_vertex(vtx[j], v);
-  const CT w = valid ? (omega ? CT(omega[j]) : relaxation) : CT(0);
+  uint32_t fw[3] = {0, 0, 0};  // factor rows of the wave's patch (BATCH)
+  CT w;
+  if constexpr(BATCH)
+  {
+    const int jw = min(j0, patch_end - 1);  // wave-uniform
+    const uint32_t pv = vtx[jw];
+#pragma unroll
+    for(int d = 0; d < 3; ++d) fw[d] = fac[size_t(jw) * 3 + d];
+    unpack_vertex(pv, v);
+  }
+  else
+  {
+    // one wave per patch with the matrices not in scalar registers (FP64): the
+    // even-odd products take the half matrices straight from scalar loads, so
+    // only patches without mirror symmetry stage the full matrices in LDS
+    bool stage_S = true;
+    if constexpr(WAVE_PATCH && !S_SCALAR)
+    {
+      const int jw = min(j0, patch_end - 1);
+      stage_S = !(eo_tab && (eo_tab[fac[size_t(jw) * DIM]] & eo_tab[fac[size_t(jw) * DIM + 1]] &
+                             eo_tab[fac[size_t(jw) * DIM + 2]]) != 0u);
+    }
+    if constexpr(!S_SCALAR)
+      for(int idx = threadIdx.x; stage_S && idx < PPB * DIM * K * K; idx += blockDim.x)
+      {
+        const int q = idx / (DIM * K * K), r = idx % (DIM * K * K);
+        const int d = r / (K * K), e = r % (K * K);
+        const int jj = j0 + q;
+        Ss[q][d][e] = jj < patch_end ? S_tab[size_t(fac[size_t(jj) * DIM + d]) * K * K + e] : CT(0);
+      }
+    for(int idx = threadIdx.x; idx < PPB * DIM * K; idx += blockDim.x)
+    {
+      const int q = idx / (DIM * K), r = idx % (DIM * K);
+      const int d = r / K, e = r % K;
+      const int jj = j0 + q;
+      ls[q][d][e] = jj < patch_end ? lam_tab[size_t(fac[size_t(jj) * DIM + d]) * K + e] : CT(1);
+    }
+    if(valid) unpack_vertex(vtx[j], v);
+    w = valid ? (omega ? CT(omega[j]) : relaxation) : CT(0);
+  }
 
   const int ty = DIM > 1 ? t % K : 0;
   const int tz = DIM > 2 ? t / K : 0;
@@ -199,7 +270,19 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   const int64_t c_lo = patch_cell<DIM, NP>(v, n, 0, ty, tz);
   const int lyz = patch_local<DIM, NP>(0, ty, tz);
   CT g[K], r[K], o[K];
-  if(valid)
+  // BATCH: this lane's entries of the patch tables, staged in LDS after the
+  // coupling stage (their loads go out with those of b and the neighbours)
+  CT sv[3], lv = CT(1);
+  if constexpr(BATCH)
+  {
+    if constexpr(!S_SCALAR)
+#pragma unroll
+      for(int d = 0; d < 3; ++d) sv[d] = S_tab[size_t(fw[d]) * K * K + t];
+    if(t < 3 * K) lv = lam_tab[size_t(fw[t / K]) * K + t % K];
+    load_row<NP>(b + c_lo * NDC + lyz, g);
+    load_row<NP>(b + (c_lo + 1) * NDC + lyz, g + NP);
+  }
+  else if(valid)
   {
     load_row<NP>(b + c_lo * NDC + lyz, g);
     load_row<NP>(b + (c_lo + 1) * NDC + lyz, g + NP);
@@ -219,48 +302,56 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
       t1[i] = 0;
       t2[i] = 0;
     }
-    if(valid && v[0] - 1 > 0)
+    if constexpr(BATCH)
     {
-      CT xl[NP];
-      load_row<NP>(x + (c_lo - 1) * NDC + lyz, xl);
+      // the outer neighbour rows of this lane: the x-neighbours of its line and
+      // one row of each of the two y- and z-neighbour planes; absent neighbours
+      // read the patch's own cells (in bounds) and are discarded below
+      const bool hl = v[0] - 1 > 0, hr = v[0] + 1 < n;
+      const bool hy = by == 0 ? (v[1] - 1 > 0) : (v[1] + 1 < n);
+      const bool hz = bz == 0 ? (v[2] - 1 > 0) : (v[2] + 1 < n);
+      CT xl[NP], xr[NP], ry[2][NP], rz[2][NP];
+      load_row<NP>(x + (hl ? c_lo - 1 : c_lo) * NDC + lyz, xl);
+      load_row<NP>(x + (hr ? c_lo + 2 : c_lo + 1) * NDC + lyz, xr);
+      const int64_t sy = hy ? (by == 0 ? -int64_t(n) : int64_t(n)) : 0;
+      const int64_t sz = hz ? (bz == 0 ? -nn : nn) : 0;
+#pragma unroll
+      for(int cx = 0; cx < 2; ++cx)
+      {
+        // y-plane lz, row ly; z-plane ly, row lz
+        load_row<NP>(x + (c_lo + cx + sy) * NDC + NP * NP * lz + NP * ly, ry[cx]);
+        load_row<NP>(x + (c_lo + cx + sz) * NDC + NP * ly + NP * NP * lz, rz[cx]);
+      }
+      // z_0 = F x_out along x, z_1 / z_2 = the y- / z-neighbour planes contracted
+      // along the face normal, from the rows loaded above.  The contractions run
+      // for absent neighbours too and are discarded by selection: a branch would
+      // let the compiler move the loads into it, behind the waits of the others.
+      const int lane0 = int(threadIdx.x) & ~(3 << 3);
 #pragma unroll
       for(int i = 0; i < NP; ++i)
       {
-        CT s = 0;
+        CT s = 0, s2 = 0;
 #pragma unroll
-        for(int jj = 0; jj < NP; ++jj) s += tab.Fpm[i * NP + jj] * xl[jj];
-        z0[i] = s;
+        for(int jj = 0; jj < NP; ++jj)
+        {
+          s += tab.Fpm[i * NP + jj] * xl[jj];
+          s2 += tab.Fmp[i * NP + jj] * xr[jj];
+        }
+        z0[i] = hl ? s : CT(0);
+        z0[NP + i] = hr ? s2 : CT(0);
       }
-    }
-    if(valid && v[0] + 1 < n)
-    {
-      CT xr[NP];
-      load_row<NP>(x + (c_lo + 2) * NDC + lyz, xr);
-#pragma unroll
-      for(int i = 0; i < NP; ++i)
       {
-        CT s = 0;
-#pragma unroll
-        for(int jj = 0; jj < NP; ++jj) s += tab.Fmp[i * NP + jj] * xr[jj];
-        z0[NP + i] = s;
-      }
-    }
-    // z_1 (y-neighbour outside the patch) then t1 = M_x z_1 per x-cell
-    {
-      const bool has = by == 0 ? (v[1] - 1 > 0) : (v[1] + 1 < n);
-      const CT *F = by == 0 ? tab.Fpm : tab.Fmp;
-      if(valid && has)
-      {
-        const int64_t shift = by == 0 ? -int64_t(n) : int64_t(n);
+        // the quad's four lanes (ly = 0..3) hold the four rows of the plane
+        const CT *F = by == 0 ? tab.Fpm : tab.Fmp;
         CT z1[K];
 #pragma unroll
         for(int cx = 0; cx < 2; ++cx)
         {
-          // the x-y plane lz of the neighbour cell: NP rows, contiguous
           CT xn[NP * NP];
-          const VT *pn = x + (c_lo + cx + shift) * NDC + NP * NP * lz;
 #pragma unroll
-          for(int jj = 0; jj < NP; ++jj) load_row<NP>(pn + NP * jj, xn + NP * jj);
+          for(int jj = 0; jj < NP; ++jj)
+#pragma unroll
+            for(int lx = 0; lx < NP; ++lx) xn[lx + NP * jj] = quad_bcast_rt(ry[cx][lx], jj);
 #pragma unroll
           for(int lx = 0; lx < NP; ++lx)
           {
@@ -278,26 +369,21 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
             CT s = 0;
 #pragma unroll
             for(int jj = 0; jj < NP; ++jj) s += tab.M[i * NP + jj] * z1[cx * NP + jj];
-            t1[cx * NP + i] = s;
+            t1[cx * NP + i] = hy ? s : CT(0);
           }
       }
-    }
-    if(DIM > 2)
-    {
-      const bool has = bz == 0 ? (v[2] - 1 > 0) : (v[2] + 1 < n);
-      const CT *F = bz == 0 ? tab.Fpm : tab.Fmp;
-      if(valid && has)
       {
-        const int64_t shift = bz == 0 ? -nn : nn;
+        // the lanes of line (ty, NP bz + kk), kk = 0..3, hold the rows kk
+        const CT *F = bz == 0 ? tab.Fpm : tab.Fmp;
         CT z2[K];
 #pragma unroll
         for(int cx = 0; cx < 2; ++cx)
         {
-          // the x-z plane ly of the neighbour cell: NP rows at stride NP^2
           CT xn[NP * NP];
-          const VT *pn = x + (c_lo + cx + shift) * NDC + NP * ly;
 #pragma unroll
-          for(int kk = 0; kk < NP; ++kk) load_row<NP>(pn + NP * NP * kk, xn + NP * kk);
+          for(int kk = 0; kk < NP; ++kk)
+#pragma unroll
+            for(int lx = 0; lx < NP; ++lx) xn[lx + NP * kk] = lane_read(rz[cx][lx], lane0 | (kk << 3));
 #pragma unroll
           for(int lx = 0; lx < NP; ++lx)
           {
@@ -315,8 +401,111 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
             CT s = 0;
 #pragma unroll
             for(int jj = 0; jj < NP; ++jj) s += tab.M[i * NP + jj] * z2[cx * NP + jj];
-            t2[cx * NP + i] = s;
+            t2[cx * NP + i] = hz ? s : CT(0);
           }
+      }
+    }
+    else
+    {
+      if(valid && v[0] - 1 > 0)
+      {
+        CT xl[NP];
+        load_row<NP>(x + (c_lo - 1) * NDC + lyz, xl);
+#pragma unroll
+        for(int i = 0; i < NP; ++i)
+        {
+          CT s = 0;
+#pragma unroll
+          for(int jj = 0; jj < NP; ++jj) s += tab.Fpm[i * NP + jj] * xl[jj];
+          z0[i] = s;
+        }
+      }
+      if(valid && v[0] + 1 < n)
+      {
+        CT xr[NP];
+        load_row<NP>(x + (c_lo + 2) * NDC + lyz, xr);
+#pragma unroll
+        for(int i = 0; i < NP; ++i)
+        {
+          CT s = 0;
+#pragma unroll
+          for(int jj = 0; jj < NP; ++jj) s += tab.Fmp[i * NP + jj] * xr[jj];
+          z0[NP + i] = s;
+        }
+      }
+      // z_1 (y-neighbour outside the patch) then t1 = M_x z_1 per x-cell
+      {
+        const bool has = by == 0 ? (v[1] - 1 > 0) : (v[1] + 1 < n);
+        const CT *F = by == 0 ? tab.Fpm : tab.Fmp;
+        if(valid && has)
+        {
+          const int64_t shift = by == 0 ? -int64_t(n) : int64_t(n);
+          CT z1[K];
+#pragma unroll
+          for(int cx = 0; cx < 2; ++cx)
+          {
+            // the x-y plane lz of the neighbour cell: NP rows, contiguous
+            CT xn[NP * NP];
+            const VT *pn = x + (c_lo + cx + shift) * NDC + NP * NP * lz;
+#pragma unroll
+            for(int jj = 0; jj < NP; ++jj) load_row<NP>(pn + NP * jj, xn + NP * jj);
+#pragma unroll
+            for(int lx = 0; lx < NP; ++lx)
+            {
+              CT s = 0;
+#pragma unroll
+              for(int jj = 0; jj < NP; ++jj) s += F[ly * NP + jj] * xn[lx + NP * jj];
+              z1[cx * NP + lx] = s;
+            }
+          }
+#pragma unroll
+          for(int cx = 0; cx < 2; ++cx)
+#pragma unroll
+            for(int i = 0; i < NP; ++i)
+            {
+              CT s = 0;
+#pragma unroll
+              for(int jj = 0; jj < NP; ++jj) s += tab.M[i * NP + jj] * z1[cx * NP + jj];
+              t1[cx * NP + i] = s;
+            }
+        }
+      }
+      if(DIM > 2)
+      {
+        const bool has = bz == 0 ? (v[2] - 1 > 0) : (v[2] + 1 < n);
+        const CT *F = bz == 0 ? tab.Fpm : tab.Fmp;
+        if(valid && has)
+        {
+          const int64_t shift = bz == 0 ? -nn : nn;
+          CT z2[K];
+#pragma unroll
+          for(int cx = 0; cx < 2; ++cx)
+          {
+            // the x-z plane ly of the neighbour cell: NP rows at stride NP^2
+            CT xn[NP * NP];
+            const VT *pn = x + (c_lo + cx + shift) * NDC + NP * ly;
+#pragma unroll
+            for(int kk = 0; kk < NP; ++kk) load_row<NP>(pn + NP * NP * kk, xn + NP * kk);
+#pragma unroll
+            for(int lx = 0; lx < NP; ++lx)
+            {
+              CT s = 0;
+#pragma unroll
+              for(int kk = 0; kk < NP; ++kk) s += F[lz * NP + kk] * xn[lx + NP * kk];
+              z2[cx * NP + lx] = s;
+            }
+          }
+#pragma unroll
+          for(int cx = 0; cx < 2; ++cx)
+#pragma unroll
+            for(int i = 0; i < NP; ++i)
+            {
+              CT s = 0;
+#pragma unroll
+              for(int jj = 0; jj < NP; ++jj) s += tab.M[i * NP + jj] * z2[cx * NP + jj];
+              t2[cx * NP + i] = s;
+            }
+        }
       }
     }
     // cross-line mass contractions through LDS
@@ -369,9 +558,19 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
     }
   }
 
+  if constexpr(BATCH)
+  {
+    if constexpr(!S_SCALAR)
+#pragma unroll
+      for(int d = 0; d < 3; ++d) Ss[0][d][t] = sv[d];
+    if(t < 3 * K) (&ls[0][0][0])[t] = lv;
+    w = valid ? (omega ? CT(omega[min(j0, patch_end - 1)]) : relaxation) : CT(0);
+    __syncthreads();
+  }
   // the factor tables in LDS are complete (without the coupling stage no
   // barrier has passed since they were written)
-  if(x_is_zero) __syncthreads();
+  else if(x_is_zero)
+    __syncthreads();
   if constexpr(FD == 1)
   {
     static_assert(DIM == 3 && K == 8 && PPB == 2, "MFMA patch solve: 3D, K = 8, two patches per block");

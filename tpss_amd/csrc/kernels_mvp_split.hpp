@@ -82,19 +82,17 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
 
   const int t = threadIdx.x;
   const int j = patch_begin + int(blockIdx.x);  // one patch per block, grid = patches
-  for(int idx = t; idx < 3 * K * K; idx += 256)
-  {
-    const int d = idx / (K * K), e = idx % (K * K);
-    Ss[d][e] = S_tab[size_t(fac[size_t(j) * 3 + d]) * K * K + e];
-  }
-  if(t < 3 * K)
-  {
-    const int d = t / K, e = t % K;
-    ls[d][e] = lam_tab[size_t(fac[size_t(j) * 3 + d]) * K + e];
-  }
+  // The pass takes as long as one patch's chain of memory round trips: the
+  // patch's vertex and factor rows (block-uniform), then every other load of
+  // the pass -- the factor tables, b, the outer neighbour rows -- in one batch,
+  // with the tables staged in LDS after the neighbour contractions.  Absent
+  // neighbours read the patch's own cells (in bounds) and are discarded by
+  // selection, so that no load waits behind a branch.
   int v[3] = {1, 1, 1};
   unpack_vertex(vtx[j], v);
-  const CT w = omega ? CT(omega[j]) : relaxation;
+  uint32_t fw[3];
+#pragma unroll
+  for(int d = 0; d < 3; ++d) fw[d] = fac[size_t(j) * 3 + d];
 
   const int line = t >> 2, q = t & 3, hq = q & 1;
   const int ty = line & 7, tz = line >> 3;
@@ -104,25 +102,44 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   const int64_t c_lo = patch_cell<3, NP>(v, n, 0, ty, tz);
   const int64_t cell = c_lo + cx;
   const int lyz = NP * ly + NP * NP * lz;
+  CT sv = CT(0), lv = CT(1);
+  if(t < 3 * K * K) sv = S_tab[size_t(fw[t / (K * K)]) * K * K + t % (K * K)];
+  if(t < 3 * K) lv = lam_tab[size_t(fw[t / K]) * K + t % K];
   CT g[2];
   load_pair(b + cell * NDC + lyz + lx0, g);
 
+  CT z0[2] = {CT(0), CT(0)}, t1[2] = {CT(0), CT(0)}, t2[2] = {CT(0), CT(0)};
   if(!x_is_zero)
   {
-    CT z0[2] = {CT(0), CT(0)}, t1[2] = {CT(0), CT(0)}, t2[2] = {CT(0), CT(0)};
     // z_0: the x-neighbour of this thread's x-cell (left of cell 0, right of cell 1)
-    if(cx == 0 ? (v[0] - 1 > 0) : (v[0] + 1 < n))
+    const bool hx = cx == 0 ? (v[0] - 1 > 0) : (v[0] + 1 < n);
+    CT xr[NP];
+    load_row<NP>(x + (hx ? (cx == 0 ? c_lo - 1 : c_lo + 2) : cell) * NDC + lyz, xr);
+    // the rows along the face normal of the y- and z-neighbour planes at this
+    // line's position (two entries each)
+    bool has[2];
+    CT rows[2][NP][2];
+#pragma unroll
+    for(int dir = 1; dir < 3; ++dir)
+    {
+      const int bside = dir == 1 ? by : bz;
+      has[dir - 1] = bside == 0 ? (v[dir] - 1 > 0) : (v[dir] + 1 < n);
+      const int64_t step = dir == 1 ? int64_t(n) : nn;
+      const VT *pn = x + (cell + (has[dir - 1] ? (bside == 0 ? -step : step) : 0)) * NDC +
+                     (dir == 1 ? NP * NP * lz : NP * ly);
+      const int rs = dir == 1 ? NP : NP * NP;  // stride of the rows along the normal
+#pragma unroll
+      for(int jj = 0; jj < NP; ++jj) load_pair(pn + rs * jj + lx0, rows[dir - 1][jj]);
+    }
     {
       const CT *F = cx == 0 ? tab.Fpm : tab.Fmp;
-      CT xr[NP];
-      load_row<NP>(x + (cx == 0 ? c_lo - 1 : c_lo + 2) * NDC + lyz, xr);
 #pragma unroll
       for(int k = 0; k < 2; ++k)
       {
         CT s = 0;
 #pragma unroll
         for(int jj = 0; jj < NP; ++jj) s += F[(lx0 + k) * NP + jj] * xr[jj];
-        z0[k] = s;
+        z0[k] = hx ? s : CT(0);
       }
     }
     // z_1 (y) and z_2 (z): the neighbour plane contracted along the face
@@ -132,25 +149,16 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
     for(int dir = 1; dir < 3; ++dir)
     {
       const int bside = dir == 1 ? by : bz, lpos = dir == 1 ? ly : lz;
-      const bool has = bside == 0 ? (v[dir] - 1 > 0) : (v[dir] + 1 < n);
       const CT *F = bside == 0 ? tab.Fpm : tab.Fmp;
       CT z[2] = {CT(0), CT(0)};
-      if(has)
-      {
-        const int64_t step = dir == 1 ? int64_t(n) : nn;
-        const VT *pn = x + (cell + (bside == 0 ? -step : step)) * NDC +
-                       (dir == 1 ? NP * NP * lz : NP * ly);
-        const int rs = dir == 1 ? NP : NP * NP;  // stride of the rows along the normal
 #pragma unroll
-        for(int jj = 0; jj < NP; ++jj)
-        {
-          CT row[2];
-          load_pair(pn + rs * jj + lx0, row);
-          const CT f = F[lpos * NP + jj];
-          z[0] += f * row[0];
-          z[1] += f * row[1];
-        }
+      for(int jj = 0; jj < NP; ++jj)
+      {
+        const CT f = F[lpos * NP + jj];
+        z[0] += f * rows[dir - 1][jj][0];
+        z[1] += f * rows[dir - 1][jj][1];
       }
+      if(!has[dir - 1]) z[0] = z[1] = CT(0);
       const CT zo0 = quad_perm<0xB1>(z[0]), zo1 = quad_perm<0xB1>(z[1]);  // lane ^ 1
       CT zc[NP];
 #pragma unroll
@@ -171,6 +179,14 @@ k_patch_mvp_split(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
           t2[k] = s;
       }
     }
+  }
+  // the patch tables, published by the next barrier
+  if(t < 3 * K * K) (&Ss[0][0])[t] = sv;
+  if(t < 3 * K) (&ls[0][0])[t] = lv;
+  const CT w = omega ? CT(omega[j]) : relaxation;
+
+  if(!x_is_zero)
+  {
     // cross-line mass contractions:  g -= M_y M_x z_2 + M_z (M_x z_1 + M_y z_0)
     CT *Z = A1;
     CT *T2 = A2;
