@@ -35,7 +35,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps first: the clocks settle within about 10 ms of work")
     ap.add_argument("--n", type=int, default=64, help="cells per direction")
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
