@@ -202,8 +202,10 @@ int tpss_smoother_fit_errors(const tpss_smoother *sm, double *out);
 int tpss_smoother_get_timings(const tpss_smoother *sm, tpss_time_info *out, int *n);
 int tpss_smoother_set_timing(tpss_smoother *sm, int enabled);
 /* AssertIsFinite(dst.l2_norm()) (schwarz_preconditioner.templates.h:238, debug-only
- * in the reference), on by default: each apply entry point ends with one pass
- * over dst that sets a flag in pinned host memory when it meets NaN/Inf.
+ * in the reference), on by default: the values written to dst are checked and a
+ * flag in pinned host memory is set when one is NaN/Inf -- by the residual-free
+ * color passes themselves where they write all of dst (Cartesian multiplicative
+ * vmult / Tvmult / step), otherwise by one pass over dst at the end of the call.
  * mode 1 (default): no host synchronisation; the flag is reported as
  * TPSS_E_NONFINITE by the next apply entry point of this smoother or by
  * tpss_smoother_finite_status.  mode 2: the call synchronises its stream and

@@ -270,16 +270,23 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
     A0[t] = v;
     A1[t] = g;
   }
-  for(int it = t; it < (NFACE + NCORN) * DIM; it += NT)
+  for(int it = t; it < (NFACE + 1) * DIM; it += NT)
   {
     if(it >= NFACE * DIM)
     {
-      // A_S = sum over subsets R of S of (-1)^{|S \ R|} X_R
-      const int S = it / DIM - NFACE, a = it % DIM;
-      double s = 0.0;
-      for(int R = 0; R < NCORN; ++R)
-        if((R & S) == R) s += (__popc(unsigned(S ^ R)) & 1) ? -X[R][a] : X[R][a];
-      C[S][a] = T(s);
+      // coordinate a of A_S = sum over subsets R of S of (-1)^{|S \ R|} X_R: one
+      // subtraction stage per direction
+      const int a = it - NFACE * DIM;
+      double c[NCORN];
+#pragma unroll
+      for(int R = 0; R < NCORN; ++R) c[R] = X[R][a];
+#pragma unroll
+      for(int e = 0; e < DIM; ++e)
+#pragma unroll
+        for(int S = 0; S < NCORN; ++S)
+          if((S >> e) & 1) c[S] -= c[S ^ (1 << e)];
+#pragma unroll
+      for(int S = 0; S < NCORN; ++S) C[S][a] = T(c[S]);
       continue;
     }
     // face f, coordinate a: Y = the face's corners, E / En = the edges along the
@@ -351,26 +358,43 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
       B2[t] = b2;
     }
   }
-  for(int o = t; o < NFACE * 2 * NF; o += NT)
   {
-    const int f = o / (2 * NF), sd = (o / NF) % 2, m = o % NF;
-    if(sd == 1 && !interior(f)) continue;
-    const int d = f / 2, side = f % 2;
-    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
-    const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
-    const int base = i1 * stride(e1) + (DIM == 3 ? i2 * stride(e2) : 0);
-    const T *src = sd == 0 ? U : UN[f];
-    const int fs = sd == 0 ? side : 1 - side;
-    T a = 0, bs = 0;
+    auto normal_trace = [&](int d, int side, int sd, int m) {
+      const int f = 2 * d + side;
+      if(sd == 1 && !interior(f)) return;
+      const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+      const int i1 = m % NP, i2 = DIM == 3 ? m / NP : 0;
+      const int base = i1 * stride(e1) + (DIM == 3 ? i2 * stride(e2) : 0);
+      const T *src = sd == 0 ? U : UN[f];
+      const int fs = sd == 0 ? side : 1 - side;
+      T a = 0, bs = 0;
 #pragma unroll
-    for(int id = 0; id < NP; ++id)
+      for(int id = 0; id < NP; ++id)
+      {
+        const T u = src[base + id * stride(d)];
+        a += tl.f[fs][id] * u;
+        bs += tl.fd[fs][id] * u;
+      }
+      PA[f][sd][m] = a;
+      PB[f][sd][m] = bs;
+    };
+    if constexpr(DIM == 3 && 4 * NF <= NT && NT < 8 * NF)
     {
-      const T u = src[base + id * stride(d)];
-      a += tl.f[fs][id] * u;
-      bs += tl.fd[fs][id] * u;
+      // (3D Q3, Q4) the items (side, sd, m) of one normal direction fill a pass:
+      // one pass per direction, which makes d and the strides constants
+      if(t < 4 * NF)
+      {
+        const int side = t / (2 * NF), sd = (t / NF) % 2, m = t % NF;
+#pragma unroll
+        for(int d = 0; d < DIM; ++d) normal_trace(d, side, sd, m);
+      }
     }
-    PA[f][sd][m] = a;
-    PB[f][sd][m] = bs;
+    else
+      for(int o = t; o < NFACE * 2 * NF; o += NT)
+      {
+        const int f = o / (2 * NF);
+        normal_trace(f / 2, f % 2, (o / NF) % 2, o % NF);
+      }
   }
   __syncthreads();
 
