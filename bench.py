@@ -148,7 +148,8 @@ def operators_section(t, n=64, reps=10):
             x = t.DeviceVector(lv.n_dofs, prec)
             y = t.DeviceVector(lv.n_dofs, prec)
             t.fill_random(x, 1)
-            lv.vmult(y, x)
+            for _ in range(reps):  # warm-up: the clocks settle within about 10 ms of work
+                lv.vmult(y, x)
             t.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):

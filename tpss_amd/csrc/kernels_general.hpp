@@ -188,21 +188,25 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
   const int t = threadIdx.x;
   const bool act = t < NDC;
   const int ts = act ? t : 0;
-  int cc[3] = {0, 0, 0};
+  // faces with a neighbour (bit f = 2 d + side) and the index of the cell's first
+  // vertex (vertex indices fit in 32 bits: (n + 1)^3 < 2^31 for n <= 1000)
+  int imask = 0, vbase = 0;
   {
     int r = int(cell);  // cells < n^3 <= 1000^3 < 2^31
+    int vst = 1;
 #pragma unroll
     for(int d = 0; d < DIM; ++d)
     {
-      cc[d] = r % n;
+      const int cd = r % n;
       r /= n;
+      imask |= (cd > 0 ? 1 : 0) << (2 * d);
+      imask |= (cd < n - 1 ? 1 : 0) << (2 * d + 1);
+      vbase += cd * vst;
+      vst *= n + 1;
     }
   }
   const int ix = ts % NP, iy = DIM > 1 ? (ts / NP) % NP : 0, iz = DIM > 2 ? ts / (NP * NP) : 0;
-  auto interior = [&](int f) {
-    const int d = f / 2;
-    return (f % 2) == 1 ? cc[d] < n - 1 : cc[d] > 0;
-  };
+  auto interior = [&](int f) { return ((imask >> f) & 1) != 0; };
   auto nb_stride = [&](int d) {
     int64_t st = 1;
     for(int e = 0; e < d; ++e) st *= n;
@@ -228,9 +232,8 @@ k_sipg_general(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restric
       }
   }
   {
-    // vertex indices fit in 32 bits: (n + 1)^3 < 2^31 for n <= 1000
     int vs[3] = {1, n + 1, (n + 1) * (n + 1)};
-    const int vb = cc[0] + cc[1] * vs[1] + (DIM == 3 ? cc[2] * vs[2] : 0);
+    const int vb = vbase;
     for(int it = t; it < (NCORN + NFACE * NHC) * DIM; it += NT)
     {
       const int c = it / DIM, a = it % DIM;
