@@ -794,60 +794,108 @@ k_patch_fd(const VT *__restrict__ src, VT *__restrict__ dst, CT *__restrict__ ub
   const int j = j0 + pl;
   const bool valid = pl < PPB && j < patch_end;
   const int plc = pl < PPB ? pl : 0;
+  constexpr int NT = PS::THREADS;
 
-  for(int idx = threadIdx.x; idx < PPB * DIM * K * K; idx += blockDim.x)
+  // Every load of the pass in three batches instead of one memory round trip
+  // per table entry and line element: the block's factor rows and this lane's
+  // vertex; the table entries this thread stages and its patch line (two cell
+  // rows of b, in MODE 0 also of dst -- the cells of a color's patches are
+  // disjoint, so no other patch of the launch writes them); then the LDS
+  // stores.  Loads are predicated, not branched, so that the compiler issues
+  // them together.
+  constexpr int NSE = PPB * DIM * K * K, NSI = (NSE + NT - 1) / NT;  // eigenvector entries
+  constexpr int NLE = PPB * DIM * K, NLI = (NLE + NT - 1) / NT;      // eigenvalues
+  uint32_t fs[NSI], fl[NLI];
+#pragma unroll
+  for(int k = 0; k < NSI; ++k)
   {
-    const int q = idx / (DIM * K * K), r = idx % (DIM * K * K);
-    const int d = r / (K * K), e = r % (K * K);
-    const int jj = j0 + q;
-    Ss[q][d][e] = jj < patch_end ? S_tab[size_t(fac[size_t(jj) * DIM + d]) * K * K + e] : CT(0);
+    const int idx = int(threadIdx.x) + k * NT;
+    const int q = idx / (DIM * K * K), d = (idx % (DIM * K * K)) / (K * K);
+    const bool ok = idx < NSE && j0 + q < patch_end;
+    fs[k] = fac[size_t(ok ? j0 + q : patch_begin) * DIM + (ok ? d : 0)];
   }
-  for(int idx = threadIdx.x; idx < PPB * DIM * K; idx += blockDim.x)
+#pragma unroll
+  for(int k = 0; k < NLI; ++k)
   {
-    const int q = idx / (DIM * K), r = idx % (DIM * K);
-    const int d = r / K, e = r % K;
-    const int jj = j0 + q;
-    ls[q][d][e] = jj < patch_end ? lam_tab[size_t(fac[size_t(jj) * DIM + d]) * K + e] : CT(1);
+    const int idx = int(threadIdx.x) + k * NT;
+    const int q = idx / (DIM * K), d = (idx % (DIM * K)) / K;
+    const bool ok = idx < NLE && j0 + q < patch_end;
+    fl[k] = fac[size_t(ok ? j0 + q : patch_begin) * DIM + (ok ? d : 0)];
   }
   int v[3] = {1, 1, 1};
-  if(valid) unpack_vertex(vtx[j], v);
+  unpack_vertex(vtx[valid ? j : patch_begin], v);
   const CT w = valid ? (omega ? CT(omega[j]) : relaxation) : CT(0);
-  __syncthreads();
 
   auto cell_of = [&](int i0, int i1, int i2) { return patch_cell<DIM, NP>(v, n, i0, i1, i2); };
   auto loc_of = [&](int i0, int i1, int i2) { return patch_local<DIM, NP>(i0, i1, i2); };
   auto slot_of = [&](int i0, int i1, int i2) { return patch_slot<DIM, NP>(i0, i1, i2); };
 
-  CT r[K], o[K];
+  CT sv[NSI], lv[NLI];
+#pragma unroll
+  for(int k = 0; k < NSI; ++k)
+  {
+    const int idx = int(threadIdx.x) + k * NT;
+    sv[k] = S_tab[size_t(fs[k]) * K * K + idx % (K * K)];
+  }
+#pragma unroll
+  for(int k = 0; k < NLI; ++k)
+  {
+    const int idx = int(threadIdx.x) + k * NT;
+    lv[k] = lam_tab[size_t(fl[k]) * K + idx % K];
+  }
+  // this thread's x-line (ty, tz) of the patch: row lyz of the cells c_lo, c_lo + 1
+  const int ty = DIM > 1 ? t % K : 0;
+  const int tz = DIM > 2 ? t / K : 0;
+  const int64_t c_lo = cell_of(0, ty, tz);
+  const int lyz = loc_of(0, ty, tz);
+  CT r[K], o[K], old[K];
+  load_row<NP>(src + c_lo * NDC + lyz, r);
+  load_row<NP>(src + (c_lo + 1) * NDC + lyz, r + NP);
+  if constexpr(MODE == 0)
+  {
+    load_row<NP>(dst + c_lo * NDC + lyz, old);
+    load_row<NP>(dst + (c_lo + 1) * NDC + lyz, old + NP);
+  }
+#pragma unroll
+  for(int k = 0; k < NSI; ++k)
+  {
+    const int idx = int(threadIdx.x) + k * NT;
+    const int q = idx / (DIM * K * K), rr = idx % (DIM * K * K);
+    const bool ok = idx < NSE;
+    if(ok) Ss[q][rr / (K * K)][rr % (K * K)] = j0 + q < patch_end ? sv[k] : CT(0);
+  }
+#pragma unroll
+  for(int k = 0; k < NLI; ++k)
+  {
+    const int idx = int(threadIdx.x) + k * NT;
+    const int q = idx / (DIM * K), rr = idx % (DIM * K);
+    if(idx < NLE) ls[q][rr / K][rr % K] = j0 + q < patch_end ? lv[k] : CT(1);
+  }
+  if(!valid)
+#pragma unroll
+    for(int i = 0; i < K; ++i) r[i] = CT(0);
+  __syncthreads();
+
   const CT *S0 = Ss[plc][0];
   const CT *S1 = Ss[plc][1];
   const CT *S2 = Ss[plc][DIM > 2 ? 2 : 1];
   CT *B = buf[plc];
 
-  // ---- gather x-lines: thread (y, z) = (t % K, t / K)
-  const int ty = DIM > 1 ? t % K : 0;
-  const int tz = DIM > 2 ? t / K : 0;
-#pragma unroll
-  for(int i = 0; i < K; ++i)
-    r[i] = valid ? CT(src[cell_of(i, ty, tz) * NDC + loc_of(i, ty, tz)]) : CT(0);
-
   fd_lines<DIM, K, CT>(r, o, B, S0, S1, S2, ls[plc][0], ls[plc][1], ls[plc][DIM > 2 ? 2 : 1], w,
                        t, pl < PPB);
   if(!valid) return;
-#pragma unroll
-  for(int a = 0; a < K; ++a)
+  if constexpr(MODE == 0)
   {
-    const int64_t c = cell_of(a, ty, tz);
-    const int l = loc_of(a, ty, tz);
-    if(MODE == 0)
-    {
-      VT *p = dst + c * NDC + l;
-      *p = VT(CT(*p) + o[a]);
-    }
-    else
-    {
-      ubuf[(c * (1 << DIM) + slot_of(a, ty, tz)) * NDC + l] = o[a];
-    }
+#pragma unroll
+    for(int a = 0; a < K; ++a) o[a] += old[a];
+    store_row<NP>(dst + c_lo * NDC + lyz, o);
+    store_row<NP>(dst + (c_lo + 1) * NDC + lyz, o + NP);
+  }
+  else
+  {
+    const int s0 = slot_of(0, ty, tz);
+    store_row<NP>(ubuf + (c_lo * (1 << DIM) + s0) * NDC + lyz, o);
+    store_row<NP>(ubuf + ((c_lo + 1) * (1 << DIM) + (s0 | 1)) * NDC + lyz, o + NP);
   }
 }
 
