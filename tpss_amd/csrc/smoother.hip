@@ -197,6 +197,52 @@ void fit_local_solvers(const tpss_level *lv, LocalSolvers *ls, int sweeps, const
   ls->fit_err = d_err.download();
 }
 
+// Launch order of one color's patches for the residual-free pass (3D, red-black):
+// every outer neighbour cell of a color is read by three of its patches, the
+// ones across its x-, y- and z-face.  In vertex-lexicographic order the z-reader
+// comes one plane of patches (about 500 blocks, 10 MB of traffic) after the
+// other two, too far for the 4 MB L2 of an XCD, and the hardware deals
+// consecutive blocks to the 8 XCDs in turn.  Here the (x, y) extent of the color
+// is cut into 4 x 2 column tiles, one per XCD; a tile's patches go z-slab by
+// z-slab (lexicographic inside a slab), and the tiles' lists are interleaved in
+// chunks of `ppb` patches (one block), so that block i runs on XCD i mod 8 and
+// the three readers of a cell run on one XCD a slab (about 64 blocks) apart.
+// Returns the positions [begin, end) in launch order.
+std::vector<int> xcd_launch_order(const LocalSolvers &ls, int begin, int end, int ppb)
+{
+  constexpr int kXcd = 8, kTx = 4, kTy = 2;
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+  auto block = [&](int j, int d) {
+    const int x = ls.vtx_unpacked[size_t(j) * 3 + d] - 1;
+    return (x + (x & 1)) / 2;
+  };
+  for(int j = begin; j < end; ++j)
+    for(int d = 0; d < 3; ++d)
+    {
+      lo[d] = std::min(lo[d], block(j, d));
+      hi[d] = std::max(hi[d], block(j, d));
+    }
+  std::vector<std::vector<int>> tiles(kXcd);
+  for(int j = begin; j < end; ++j)
+  {
+    const int tx = (block(j, 0) - lo[0]) * kTx / (hi[0] - lo[0] + 1);
+    const int ty = (block(j, 1) - lo[1]) * kTy / (hi[1] - lo[1] + 1);
+    tiles[size_t(tx + kTx * ty)].push_back(j);
+  }
+  for(auto &t : tiles)
+    std::stable_sort(t.begin(), t.end(), [&](int a, int b) {
+      for(int d = 2; d >= 0; --d)
+        if(block(a, d) != block(b, d)) return block(a, d) < block(b, d);
+      return false;
+    });
+  std::vector<int> order;
+  order.reserve(size_t(end - begin));
+  for(size_t at = 0; order.size() < size_t(end - begin); at += size_t(ppb))
+    for(const auto &t : tiles)
+      for(size_t q = at; q < std::min(t.size(), at + size_t(ppb)); ++q) order.push_back(t[q]);
+  return order;
+}
+
 std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv,
                                                   const tpss_smoother_desc &desc, hipStream_t s)
 {
@@ -348,6 +394,23 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv,
   }
   ls->h_host = hs;
   ls->d_fac.upload(ls->fac);
+  if(dim == 3 && ls->K == 8 && ls->ncomp == 1 && coloring == TPSS_COLOR_REDBLACK && !lv->distorted)
+  {
+    std::vector<uint32_t> vtx_l(static_cast<size_t>(P)), fac_l(static_cast<size_t>(P) * 3);
+    for(int c = 0; c < ls->n_colors; ++c)
+    {
+      const std::vector<int> order =
+        xcd_launch_order(*ls, ls->color_ptr[c], ls->color_ptr[c + 1], 1);
+      for(size_t q = 0; q < order.size(); ++q)
+      {
+        const size_t at = size_t(ls->color_ptr[c]) + q, j = size_t(order[q]);
+        vtx_l[at] = ls->vtx[j];
+        for(int d = 0; d < 3; ++d) fac_l[at * 3 + d] = ls->fac[j * 3 + d];
+      }
+    }
+    ls->d_vtx_launch.upload(vtx_l);
+    ls->d_fac_launch.upload(fac_l);
+  }
 
   // FP64 factors on the device
   const int K = ls->K, nf = ls->nfac;
@@ -527,8 +590,11 @@ void mvp_fused_pass(tpss_smoother *sm, const VT *b, VT *x, int begin, int end, b
           TPSS_HIP(hipGetLastError());
           return;
         }
+      // the XCD launch order where no per-patch damping is indexed by position
+      const bool xcd = om == nullptr && ls.d_vtx_launch.ptr != nullptr && MS::PPB == 1;
       hipLaunchKernelGGL((k_patch_mvp_fused<DIM, K, VT, CT>), dim3(grid), dim3(MS::THREADS), 0, s,
-                         b, x, tab, ls.d_vtx.ptr, ls.d_fac.ptr, S, lam, ls.d_eo.ptr, om, relax,
+                         b, x, tab, xcd ? ls.d_vtx_launch.ptr : ls.d_vtx.ptr,
+                         xcd ? ls.d_fac_launch.ptr : ls.d_fac.ptr, S, lam, ls.d_eo.ptr, om, relax,
                          begin, end, ls.n, x_is_zero ? 1 : 0, sm->pass_flag);
     }
     TPSS_HIP(hipGetLastError());
