@@ -228,9 +228,19 @@ def elasticity_section(t, args, n=64, p=2, mu=1.0, lam=10.0, ip=10.0):
                      "patch_dofs": sm.n_dofs_per_patch}
         del sm, lv
     if not args.no_gmg:
-        for name, mp in (("gmg_cg_f64", t.F64), ("gmg_cg_mixed_f32", t.F32)):
+        # the reference damping (omega = 1) and the learned one (the shipped
+        # elasticity network: one damping for the Cartesian patches, its deviation
+        # from 1 doubled)
+        from tpss_amd import learned_damping as ld
+        mlp = ld.load_elasticity()
+        out["learned_damping"] = float(mlp.forward(np.ones((1, 24), dtype=np.float32))[0])
+        for name, mp, net in (("gmg_cg_f64", t.F64, None), ("gmg_cg_mixed_f32", t.F32, None),
+                              ("gmg_cg_f64_learned", t.F64, mlp),
+                              ("gmg_cg_mixed_f32_learned", t.F32, mlp)):
             g = t.GMGSolver(3, p, 2, int(np.log2(n // 2)) + 1, ip_factor=ip, mg_precision=mp,
                             variant=t.MULTIPLICATIVE, relaxation=1.0, elasticity=(mu, lam))
+            if net is not None:
+                ld.apply_to_gmg(g, net)
             b = t.DeviceVector(g.n_dofs)
             t.fill_random(b, 2)
             x = t.DeviceVector.zeros(g.n_dofs)

@@ -282,7 +282,9 @@ int tpss_mlp_train(tpss_mlp *m, const float *X, const float *target, int64_t n, 
 /* per-patch features (device float[n_patches * dim * 2^dim]) */
 int tpss_smoother_patch_features(const tpss_smoother *sm, float *dev_out, void *stream);
 /* per-patch optimal local damping omega_j = sum_e d_j^T r / sum_e d_j^T A d_j
- * (d_j = R_j^T A_j^{-1} R_j r, r = A e, e random), clamped (device float[n_patches]) */
+ * (d_j = R_j^T A_j^{-1} R_j r, r = A e, e random), clamped (device float[n_patches]);
+ * FP64 levels; on elasticity levels A_j^{-1} is the block-diagonal local solver and
+ * the products run over all component blocks of the patch */
 int tpss_smoother_optimal_damping(tpss_smoother *sm, int n_samples, uint64_t seed,
                                   double omega_min, double omega_max, float *dev_out);
 /* per-patch largest eigenvalue of M_j^{-1} A_j (M_j: the patch's local solver,

@@ -46,6 +46,24 @@ def test_optimal_damping(distorted):
         np.testing.assert_allclose(w, 1.0, rtol=1e-6)
 
 
+def test_optimal_damping_elasticity():
+    """Block-diagonal elasticity local solvers (3D Q2, mu 1, lambda 10): the line-search
+    damping over all component blocks of a patch, against the oracle's brute force."""
+    t = _tpss()
+    dim, p, n = 3, 2, 4
+    lv = t.Level(dim, p, n, ip_factor=10.0, elasticity=(1.0, 10.0))
+    sm = t.SchwarzSmoother(lv, variant=t.MULTIPLICATIVE)
+    w = sm.optimal_damping(n_samples=2, seed=5, omega_min=0.0, omega_max=10.0)
+    ref_sm = o.Smoother(dim, p, n, ip=10.0, variant=o.MULTIPLICATIVE, coloring=o.COLOR_REDBLACK,
+                        elasticity=(1.0, 10.0))
+    ref = mo.optimal_damping(ref_sm, 2, 5, lo=0.0, hi=10.0)
+    np.testing.assert_allclose(w, ref, rtol=2e-6)
+    # the block-diagonal solver drops the lambda coupling of the components: not exact
+    assert np.abs(w - 1.0).max() > 1e-3
+    f = sm.patch_features()
+    assert np.abs(f - f[0]).max() <= 1e-6 * np.abs(f).max()  # Cartesian: one feature vector
+
+
 def test_mlp_forward_and_gradient():
     t = _tpss()
     mlp = t.DampingMLP(24, 64, 0.2, 1.2, seed=4)
@@ -112,6 +130,24 @@ def test_shipped_model_beats_the_best_constant_damping(pattern):
         assert it < best_it, (it, rate, consts)
     # the reference's fixed damping 1 (TPSS.h:120-163) is far behind
     assert consts[1.0][0] < 0 or consts[1.0][0] >= it + 3, consts[1.0]
+
+
+def test_shipped_elasticity_model_beats_the_reference_damping():
+    """The elasticity network (tpss_amd/models/damping_elasticity_dgq2_3d.npy, trained on
+    the 32^3 and 48^3 levels' patches) as damping of the block-diagonal multiplicative
+    smoother: 3D SIPG DG Q2 elasticity (mu 1, lambda 10) on 64^3 cells, mixed-precision
+    GMG-CG to 1e-8 needs fewer iterations than with the reference's omega = 1, and its
+    V-cycle contracts at least as fast as the best constant damping of a sweep."""
+    t = _tpss()
+    from tpss_amd import learned_damping as ld
+    n = 64
+    mlp = ld.load_elasticity()
+    w = float(mlp.forward(np.ones((1, 24), dtype=np.float32))[0])
+    assert 0.9 < w < 1.0, w
+    it, rate = ld.gmg_solve_elasticity(n, mlp=mlp)[:2]
+    consts = {c: ld.gmg_solve_elasticity(n, relaxation=c)[:2] for c in (1.0, 0.95, 0.9, 0.85)}
+    assert 0 < it < consts[1.0][0], (it, rate, consts)
+    assert rate <= 1.01 * min(r for _, r in consts.values()), (it, rate, consts)
 
 
 def test_local_lambda_max_matches_dense_eigenproblem():

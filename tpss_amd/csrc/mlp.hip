@@ -559,13 +559,21 @@ void optimal_damping(tpss_smoother *sm, int n_samples, uint64_t seed, double lo,
       dispatch_dim_np(dim, ls.np, [&](auto Dm, auto NPc) {
         constexpr int DIM = decltype(Dm)::value, K = 2 * decltype(NPc)::value;
         using PS = PatchShape<DIM, K>;
-        hipLaunchKernelGGL((k_patch_solve_ids<DIM, K, double>), dim3((count + PS::PPB - 1) / PS::PPB),
-                           dim3(PS::THREADS), 0, s, r.ptr, D.ptr, ls.d_vtx.ptr, ls.d_fac.ptr,
-                           ls.S64.ptr, ls.lam64.ptr, gids[c]->ptr, count, ls.n);
+        // block-diagonal local solvers (elasticity): one solve per component
+        // block, with that component's factor rows
+        for(int comp = 0; comp < ls.ncomp; ++comp)
+          hipLaunchKernelGGL((k_patch_solve_ids<DIM, K, double>),
+                             dim3((count + PS::PPB - 1) / PS::PPB), dim3(PS::THREADS), 0, s,
+                             r.ptr + comp * lv->Ns, D.ptr + comp * lv->Ns, ls.d_vtx.ptr,
+                             ls.d_fac.ptr + size_t(comp) * ls.P * dim, ls.S64.ptr, ls.lam64.ptr,
+                             gids[c]->ptr, count, ls.n);
         TPSS_HIP(hipGetLastError());
         operator_apply_any(lv, y.ptr, D.ptr, nullptr, nullptr, 0, s);
-        hipLaunchKernelGGL((k_patch_dots<DIM, K, double>), dim3(count), dim3(64), 0, s, D.ptr,
-                           r.ptr, y.ptr, ls.d_vtx.ptr, gids[c]->ptr, count, ls.n, num.ptr, den.ptr);
+        // d_j^T r and d_j^T A d_j summed over the component blocks
+        for(int comp = 0; comp < ls.ncomp; ++comp)
+          hipLaunchKernelGGL((k_patch_dots<DIM, K, double>), dim3(count), dim3(64), 0, s,
+                             D.ptr + comp * lv->Ns, r.ptr + comp * lv->Ns, y.ptr + comp * lv->Ns,
+                             ls.d_vtx.ptr, gids[c]->ptr, count, ls.n, num.ptr, den.ptr);
         TPSS_HIP(hipGetLastError());
       });
     }
@@ -782,7 +790,6 @@ int tpss_smoother_patch_features(const tpss_smoother *sm, float *dev_out, void *
 {
   return guarded([&] {
     TPSS_REQUIRE(sm && dev_out, "null argument");
-    TPSS_REQUIRE(sm->level->ncomp == 1, "learned damping: Poisson levels only");
     patch_features(sm, dev_out, as_stream(stream));
   });
 }
@@ -792,7 +799,6 @@ int tpss_smoother_optimal_damping(tpss_smoother *sm, int n_samples, uint64_t see
 {
   return guarded([&] {
     TPSS_REQUIRE(sm && dev_out && n_samples >= 1, "invalid arguments");
-    TPSS_REQUIRE(sm->level->ncomp == 1, "learned damping: Poisson levels only");
     TPSS_REQUIRE(omega_max > omega_min, "empty damping range");
     optimal_damping(sm, n_samples, seed, omega_min, omega_max, dev_out, nullptr);
   });

@@ -16,6 +16,8 @@ local solver is exact keep omega = 1 whatever g is.
 
     python -m tpss_amd.learned_damping --out tpss_amd/models/damping_dgq3_3d.npy
     python -m tpss_amd.learned_damping --out tpss_amd/models/damping_dgq3_3d.npy --calibrate
+    python -m tpss_amd.learned_damping --elasticity
+    python -m tpss_amd.learned_damping --elasticity --calibrate
 """
 import argparse
 import os
@@ -123,6 +125,68 @@ def calibrate_gain(path=DEFAULT_MODEL, n=64, meshes=((0.2, "half", 201), (0.2, "
     return best, res
 
 
+# Linear elasticity (3D SIPG DG Q2, Lame mu = 1, lambda = 10, ip 10, Cartesian
+# meshes): the block-diagonal local solvers drop the lambda coupling of the
+# components, so the line-search damping of a patch is below 1 (about 0.97).  The
+# features of a Cartesian patch are all 1 (singular values scaled by n): the
+# network learns one damping for every patch, from the patches of two training
+# levels, and its deviation from 1 is amplified by a gain calibrated on a GMG-CG
+# solve, as for the Poisson network.
+ELASTICITY_MODEL = os.path.join(HERE, "models", "damping_elasticity_dgq2_3d.npy")
+ELASTICITY = (1.0, 10.0)
+ELASTICITY_IP = 10.0
+ELASTICITY_TRAIN_SIZES = (32, 48)
+# gain with the smallest GMG-CG reduction rate on the 32^3 level (mixed precision)
+# over 1, 1.5, 2, 2.5, 3, 4 (calibrate_gain_elasticity)
+ELASTICITY_GAIN = 2.0
+
+
+def elasticity_samples(n, n_samples=4, seed=7):
+    """(features, optimal damping) of every patch of the n^3 elasticity level."""
+    lv = Level(3, 2, n, ip_factor=ELASTICITY_IP, precision=F64, elasticity=ELASTICITY)
+    sm = SchwarzSmoother(lv, variant=MULTIPLICATIVE, coloring=COLOR_REDBLACK, relaxation=1.0)
+    X = sm.patch_features()
+    y = sm.optimal_damping(n_samples=n_samples, seed=seed, omega_min=DAMPING_RANGE[0],
+                           omega_max=DAMPING_RANGE[1])
+    return X, y
+
+
+def generate_elasticity(sizes=ELASTICITY_TRAIN_SIZES, n_samples=4):
+    Xs, ys = [], []
+    for i, n in enumerate(sizes):
+        X, y = elasticity_samples(n, n_samples=n_samples, seed=7 + 100 * i)
+        Xs.append(X)
+        ys.append(y)
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+def load_elasticity(path=ELASTICITY_MODEL, gain=ELASTICITY_GAIN):
+    return load(path, gain=gain)
+
+
+def gmg_solve_elasticity(n, mlp=None, relaxation=1.0, precision=F32, rel_tol=1e-8):
+    """GMG-CG (multiplicative red-black block-diagonal smoother, 2^3 coarse cells)
+    for the n^3 elasticity problem with the network's damping or a constant one:
+    (iterations, reduction rate, seconds)"""
+    levels = int(np.log2(n // 2)) + 1
+    g = GMGSolver(3, 2, 2, levels, ip_factor=ELASTICITY_IP, mg_precision=precision,
+                  variant=MULTIPLICATIVE, coloring=COLOR_REDBLACK, relaxation=relaxation,
+                  elasticity=ELASTICITY)
+    if mlp is not None:
+        apply_to_gmg(g, mlp)
+    b = DeviceVector(g.n_dofs)
+    fill_random(b, 2)
+    x = DeviceVector.zeros(g.n_dofs)
+    info, _ = g.solve(x, b, rel_tol=rel_tol, max_iter=100, raise_on_fail=False)
+    return info.iterations, info.reduction_rate, info.seconds
+
+
+def calibrate_gain_elasticity(path=ELASTICITY_MODEL, n=32, gains=(1.0, 1.5, 2.0, 2.5, 3.0, 4.0)):
+    res = {g: [gmg_solve_elasticity(n, mlp=load(path, gain=g))[:2]] for g in gains}
+    best = min(res, key=lambda k: res[k][0][1])
+    return best, res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_MODEL)
@@ -133,7 +197,33 @@ def main():
                     help="random errors per patch in the optimal-damping target")
     ap.add_argument("--calibrate", action="store_true",
                     help="only calibrate the deviation gain of the model at --out")
+    ap.add_argument("--elasticity", action="store_true",
+                    help="the elasticity network (3D Q2, mu 1, lambda 10) instead of the "
+                         "distorted-mesh Poisson one")
     args = ap.parse_args()
+    if args.elasticity:
+        out = args.out if args.out != DEFAULT_MODEL else ELASTICITY_MODEL
+        if args.calibrate:
+            best, res = calibrate_gain_elasticity(out)
+            for g, runs in res.items():
+                print(f"gain {g:.2f}: " + ", ".join(f"{it} CG iterations (rate {r:.4f})"
+                                                   for it, r in runs), flush=True)
+            print(f"calibrated deviation gain: {best:.2f}", flush=True)
+            return
+        t0 = time.time()
+        X, y = generate_elasticity(n_samples=args.samples)
+        t1 = time.time()
+        print(f"training data: {X.shape[0]} patches from levels {ELASTICITY_TRAIN_SIZES} in "
+              f"{t1 - t0:.1f} s; target mean {y.mean():.4f} std {y.std():.4f} min {y.min():.3f} "
+              f"max {y.max():.3f}", flush=True)
+        mlp, l0, l1 = train(X, y, epochs=args.epochs)
+        print(f"training: loss {l0:.3e} -> {l1:.3e} in {time.time() - t1:.1f} s "
+              f"(variance of the target {0.5 * y.var():.3e}); prediction "
+              f"{float(mlp.forward(X[:1])[0]):.4f}", flush=True)
+        os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+        mlp.save(out)
+        print("saved", out, flush=True)
+        return
     if args.calibrate:
         best, res = calibrate_gain(args.out)
         for g, runs in res.items():
