@@ -161,6 +161,18 @@ void operator_apply(const tpss_level *lv, VT *y, const VT *x, const VT *b, const
       TPSS_HIP(hipGetLastError());
       return;
     }
+    if constexpr(DIM == 3 && NP == 4)
+    {
+      const unsigned grid = blocks_for(lv->ncell, 16);
+      if(b)
+        hipLaunchKernelGGL((k_sipg_lines_q3<VT, CT, true>), dim3(grid), dim3(256), 0, s, x, y, b,
+                           tab, lv->n, lv->ncell);
+      else
+        hipLaunchKernelGGL((k_sipg_lines_q3<VT, CT, false>), dim3(grid), dim3(256), 0, s, x, y,
+                           b, tab, lv->n, lv->ncell);
+      TPSS_HIP(hipGetLastError());
+      return;
+    }
     if constexpr(DIM == 3)
     {
       constexpr int CPB = lines_cells_per_block<NP>();

@@ -99,6 +99,54 @@ __device__ __forceinline__ void store_row(VT *__restrict__ p, const CT *in)
   }
 }
 
+// value of v in lane SRC of this lane's quad (DPP quad_perm broadcast)
+template<int SRC>
+__device__ __forceinline__ int quad_bcast_i(int v)
+{
+  return __builtin_amdgcn_update_dpp(0, v, SRC * 0x55, 0xF, 0xF, false);
+}
+template<typename CT>
+__device__ __forceinline__ CT quad_bcast_rt(CT v, int src)
+{
+  if constexpr(std::is_same<CT, float>::value)
+  {
+    const int i = __float_as_int(v);
+    const int r = src == 0 ? quad_bcast_i<0>(i)
+                : src == 1 ? quad_bcast_i<1>(i)
+                : src == 2 ? quad_bcast_i<2>(i)
+                           : quad_bcast_i<3>(i);
+    return __int_as_float(r);
+  }
+  else
+  {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    int rl, rh;
+    if(src == 0)
+      rl = quad_bcast_i<0>(lo), rh = quad_bcast_i<0>(hi);
+    else if(src == 1)
+      rl = quad_bcast_i<1>(lo), rh = quad_bcast_i<1>(hi);
+    else if(src == 2)
+      rl = quad_bcast_i<2>(lo), rh = quad_bcast_i<2>(hi);
+    else
+      rl = quad_bcast_i<3>(lo), rh = quad_bcast_i<3>(hi);
+    return __hiloint2double(rh, rl);
+  }
+}
+
+// value of v in lane `lane` of the wave (ds_bpermute)
+template<typename CT>
+__device__ __forceinline__ CT lane_read(CT v, int lane)
+{
+  if constexpr(std::is_same<CT, float>::value)
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v)));
+  else
+  {
+    const int lo = __builtin_amdgcn_ds_bpermute(lane << 2, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(lane << 2, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+  }
+}
+
 template<int DIM, int NP, typename VT, typename CT, int CPB, bool RESIDUAL>
 __global__ void __launch_bounds__(256)
 k_sipg_cartesian(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
@@ -515,6 +563,193 @@ k_sipg_lines(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
 #pragma unroll
     for(int i = 0; i < NP; ++i) out[i] = bv[i] - out[i];
   }
+  store_row<NP>(y + cell * NDC + lofs, out);
+}
+
+// Q3 form of k_sipg_lines (NP = 4: a cell's 16 lines are 16 lanes of one wave).
+// The generic form loads the NP rows of every y- and z-neighbour plane in each of
+// the plane's NP lanes, behind a branch per side (about 20 row loads per lane,
+// one memory round trip after another).  Here every lane loads seven rows at
+// once -- its line of the cell and the row at the same position in each of the
+// six face neighbours (absent neighbours read the cell itself and are zeroed by
+// selection) -- and takes the other rows of a neighbour plane from the lanes
+// that loaded them: those of a y-plane are the lanes of its quad (DPP
+// broadcast), those of a z-plane sit 4 lanes apart (ds_bpermute).  The 1D
+// tables indexed by a lane-varying row (mass, face coupling, boundary-dependent
+// cell matrices) are read from LDS.  Same arithmetic, in the same order, as
+// k_sipg_lines.
+template<typename VT, typename CT, bool RESIDUAL>
+__global__ void __launch_bounds__(256, 4)
+k_sipg_lines_q3(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
+                const OpTab<CT, 4> tab, const int n, const int64_t ncell)
+{
+  constexpr int NP = 4, NL = 16, NDC = 64, CPB = 16;
+  static_assert(CPB * NL == 256, "16 cells of 16 lines per block");
+  __shared__ CT xs[CPB][NDC];  // own cells
+  __shared__ CT zs[CPB][NDC];  // z_0, later M_y z_0 + M_x z_1
+  __shared__ CT ws[CPB][NDC];  // M_x z_2
+  // the 1D tables in the layout of OpTab: M, D[0..3], F+-, F-+
+  constexpr int TB = int(sizeof(OpTab<CT, 4>) / sizeof(CT));
+  static_assert(TB == 7 * NL, "OpTab<CT, 4> holds seven 4 x 4 tables");
+  __shared__ CT tb[256];  // every thread stores (a guarded store would pull its load behind the rows)
+  const CT *tM = tb, *tFpm = tb + 5 * NL, *tFmp = tb + 6 * NL;
+  const int cs = threadIdx.x / NL, ln = threadIdx.x % NL;
+  const int ly = ln % NP, lz = ln / NP;
+  const int64_t cell = int64_t(blockIdx.x) * CPB + cs;
+  const bool act = cell < ncell;
+  const int64_t cc = act ? cell : ncell - 1;  // idle lanes load an existing cell
+  const int64_t nn = int64_t(n) * n;
+  const int ci = int(cc);
+  const int cx = ci % n, cy = (ci / n) % n, cz = ci / (n * n);
+  const bool hxl = cx > 0, hxr = cx < n - 1, hyl = cy > 0, hyr = cy < n - 1;
+  const bool hzl = cz > 0, hzr = cz < n - 1;
+  const int lofs = NP * ly + NL * lz;  // this line within the cell, and the row
+                                       // this lane loads from every neighbour
+  // this thread's table entry, loaded with the rows (staged in LDS below)
+  const CT tv = reinterpret_cast<const CT *>(&tab)[min(int(threadIdx.x), TB - 1)];
+  CT xo[NP], xl[NP], xr[NP], ya[NP], yb[NP], za[NP], zb[NP], bv[NP];
+  load_row<NP>(x + cc * NDC + lofs, xo);
+  load_row<NP>(x + (hxl ? cc - 1 : cc) * NDC + lofs, xl);
+  load_row<NP>(x + (hxr ? cc + 1 : cc) * NDC + lofs, xr);
+  load_row<NP>(x + (hyl ? cc - n : cc) * NDC + lofs, ya);
+  load_row<NP>(x + (hyr ? cc + n : cc) * NDC + lofs, yb);
+  load_row<NP>(x + (hzl ? cc - nn : cc) * NDC + lofs, za);
+  load_row<NP>(x + (hzr ? cc + nn : cc) * NDC + lofs, zb);
+  if constexpr(RESIDUAL) load_row<NP>(b + cc * NDC + lofs, bv);
+#pragma unroll
+  for(int i = 0; i < NP; ++i)
+  {
+    xl[i] = hxl ? xl[i] : CT(0);
+    xr[i] = hxr ? xr[i] : CT(0);
+    ya[i] = hyl ? ya[i] : CT(0);
+    yb[i] = hyr ? yb[i] : CT(0);
+    za[i] = hzl ? za[i] : CT(0);
+    zb[i] = hzr ? zb[i] : CT(0);
+    xs[cs][lofs + i] = xo[i];
+  }
+  tb[threadIdx.x] = tv;
+  __syncthreads();  // own cells and tables staged
+
+  // z_0: along x, own line and the x-neighbours' lines
+  CT z0[NP];
+  {
+    const CT *D = tb + NL * (1 + ((hxl ? 1 : 0) | (hxr ? 2 : 0)));
+#pragma unroll
+    for(int i = 0; i < NP; ++i)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int j = 0; j < NP; ++j)
+        s += D[i * NP + j] * xo[j] + tab.Fpm[i * NP + j] * xl[j] + tab.Fmp[i * NP + j] * xr[j];
+      z0[i] = s;
+    }
+  }
+  // z_1 (along y) and z_2 (along z) for every x-node of this line
+  CT z1[NP], z2[NP];
+  {
+    const CT *D = tb + NL * (1 + ((hyl ? 1 : 0) | (hyr ? 2 : 0)));
+#pragma unroll
+    for(int i = 0; i < NP; ++i)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int j = 0; j < NP; ++j) s += D[ly * NP + j] * xs[cs][i + NP * j + NL * lz];
+      z1[i] = s;
+    }
+#pragma unroll
+    for(int side = 0; side < 2; ++side)
+    {
+      const CT *F = side == 0 ? tFpm : tFmp;
+#pragma unroll
+      for(int j = 0; j < NP; ++j)
+      {
+        // row j of the neighbour's plane lz: loaded by lane j of this quad
+        const CT f = F[ly * NP + j];
+#pragma unroll
+        for(int i = 0; i < NP; ++i) z1[i] += f * quad_bcast_rt(side == 0 ? ya[i] : yb[i], j);
+      }
+    }
+  }
+  {
+    const CT *D = tb + NL * (1 + ((hzl ? 1 : 0) | (hzr ? 2 : 0)));
+#pragma unroll
+    for(int i = 0; i < NP; ++i)
+    {
+      CT s = 0;
+#pragma unroll
+      for(int k = 0; k < NP; ++k) s += D[lz * NP + k] * xs[cs][i + NP * ly + NL * k];
+      z2[i] = s;
+    }
+    const int base = int(threadIdx.x & 63) & ~(NL - 1);  // the cell's first lane
+#pragma unroll
+    for(int side = 0; side < 2; ++side)
+    {
+      const CT *F = side == 0 ? tFpm : tFmp;
+#pragma unroll
+      for(int k = 0; k < NP; ++k)
+      {
+        // row (ly, k) of the neighbour: loaded by lane (ly, k) of this cell
+        const CT f = F[lz * NP + k];
+        const int src = base + ly + NP * k;
+#pragma unroll
+        for(int i = 0; i < NP; ++i) z2[i] += f * lane_read(side == 0 ? za[i] : zb[i], src);
+      }
+    }
+  }
+  // x mass contractions in registers
+  CT m1[NP], w2[NP];
+#pragma unroll
+  for(int i = 0; i < NP; ++i)
+  {
+    CT s1 = 0, s2 = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+    {
+      s1 += tab.M[i * NP + j] * z1[j];
+      s2 += tab.M[i * NP + j] * z2[j];
+    }
+    m1[i] = s1;
+    w2[i] = s2;
+  }
+#pragma unroll
+  for(int i = 0; i < NP; ++i)
+  {
+    zs[cs][lofs + i] = z0[i];
+    ws[cs][lofs + i] = w2[i];
+  }
+  __syncthreads();
+  CT a[NP], bb[NP];
+#pragma unroll
+  for(int i = 0; i < NP; ++i)
+  {
+    CT s = m1[i], s2 = 0;
+#pragma unroll
+    for(int j = 0; j < NP; ++j)
+    {
+      const CT m = tM[ly * NP + j];
+      s += m * zs[cs][i + NP * j + NL * lz];
+      s2 += m * ws[cs][i + NP * j + NL * lz];
+    }
+    a[i] = s;
+    bb[i] = s2;
+  }
+  __syncthreads();
+#pragma unroll
+  for(int i = 0; i < NP; ++i) zs[cs][lofs + i] = a[i];
+  __syncthreads();
+  CT out[NP];
+#pragma unroll
+  for(int i = 0; i < NP; ++i)
+  {
+    CT s = bb[i];
+#pragma unroll
+    for(int k = 0; k < NP; ++k) s += tM[lz * NP + k] * zs[cs][i + NP * ly + NL * k];
+    out[i] = s;
+  }
+  if(!act) return;
+  if(RESIDUAL)
+#pragma unroll
+    for(int i = 0; i < NP; ++i) out[i] = bv[i] - out[i];
   store_row<NP>(y + cell * NDC + lofs, out);
 }
 

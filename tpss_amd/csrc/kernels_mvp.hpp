@@ -100,53 +100,6 @@ __device__ __forceinline__ void fd_mfma_pair(float (*B2)[512], const float (*S)[
 // others compute; longer lines (K = 10) hold more values per thread: 3.  FP64
 // solves unconstrained (3D K = 8: 186 VGPRs, 2 waves; forcing 3 spills and
 // measured 20 % slower).
-// value of v in lane SRC of this lane's quad (DPP quad_perm broadcast)
-template<int SRC>
-__device__ __forceinline__ int quad_bcast_i(int v)
-{
-  return __builtin_amdgcn_update_dpp(0, v, SRC * 0x55, 0xF, 0xF, false);
-}
-template<typename CT>
-__device__ __forceinline__ CT quad_bcast_rt(CT v, int src)
-{
-  if constexpr(std::is_same<CT, float>::value)
-  {
-    const int i = __float_as_int(v);
-    const int r = src == 0 ? quad_bcast_i<0>(i)
-                : src == 1 ? quad_bcast_i<1>(i)
-                : src == 2 ? quad_bcast_i<2>(i)
-                           : quad_bcast_i<3>(i);
-    return __int_as_float(r);
-  }
-  else
-  {
-    const int lo = __double2loint(v), hi = __double2hiint(v);
-    int rl, rh;
-    if(src == 0)
-      rl = quad_bcast_i<0>(lo), rh = quad_bcast_i<0>(hi);
-    else if(src == 1)
-      rl = quad_bcast_i<1>(lo), rh = quad_bcast_i<1>(hi);
-    else if(src == 2)
-      rl = quad_bcast_i<2>(lo), rh = quad_bcast_i<2>(hi);
-    else
-      rl = quad_bcast_i<3>(lo), rh = quad_bcast_i<3>(hi);
-    return __hiloint2double(rh, rl);
-  }
-}
-
-// value of v in lane `lane` of the wave (ds_bpermute)
-template<typename CT>
-__device__ __forceinline__ CT lane_read(CT v, int lane)
-{
-  if constexpr(std::is_same<CT, float>::value)
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v)));
-  else
-  {
-    const int lo = __builtin_amdgcn_ds_bpermute(lane << 2, __double2loint(v));
-    const int hi = __builtin_amdgcn_ds_bpermute(lane << 2, __double2hiint(v));
-    return __hiloint2double(hi, lo);
-  }
-}
 
 template<int K, typename CT, int FD>
 constexpr int mvp_min_waves()
