@@ -99,11 +99,12 @@ __device__ __forceinline__ void store_row(VT *__restrict__ p, const CT *in)
   }
 }
 
-// value of v in lane SRC of this lane's quad (DPP quad_perm broadcast)
+// value of v in lane SRC of this lane's quad (DPP quad_perm broadcast; every
+// lane has a source, so no old value has to be set first)
 template<int SRC>
 __device__ __forceinline__ int quad_bcast_i(int v)
 {
-  return __builtin_amdgcn_update_dpp(0, v, SRC * 0x55, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, SRC * 0x55, 0xF, 0xF, true);
 }
 template<typename CT>
 __device__ __forceinline__ CT quad_bcast_rt(CT v, int src)
