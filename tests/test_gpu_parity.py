@@ -324,6 +324,31 @@ def test_mvp_fused_and_residual_paths(dim, p, n, relax, rev):
         assert rel_err(host(d), expect) < 1e-12, fused
 
 
+@pytest.mark.parametrize("relax,rev,prec", [(1.0, False, 0), (0.8, True, 0), (1.0, False, 1)])
+def test_fine_level_color_passes_against_the_oracle(relax, rev, prec):
+    """3D Q3 on 23^3 cells: every color holds 665 or 666 patches, so the step runs the
+    one-wave color pass (the batched FP64 form, the FP32 form) in the XCD launch
+    order, as on the 64^3 level (smaller levels run the four-wave pass); against the
+    oracle's step and sweep, which recompute the full residual before every color."""
+    t = _tpss()
+    n = 23
+    lv = t.Level(3, 3, n, precision=prec)
+    sm = t.SchwarzSmoother(lv, variant=t.MULTIPLICATIVE, relaxation=relax, reverse=rev,
+                           local_precision=prec)
+    ref = o.Smoother(3, 3, n, variant=o.MULTIPLICATIVE, coloring=o.COLOR_REDBLACK,
+                     relaxation=relax, reverse=rev, threads=8)
+    assert np.diff(ref.color_ptr()).min() > 512
+    b = o.splitmix_uniform(21, ref.N)
+    x0 = o.splitmix_uniform(22, ref.N)
+    tol = 1e-12 if prec == t.F64 else 2e-5
+    x = dev(x0, prec)
+    sm.step(x, dev(b, prec))
+    assert rel_err(host(x), ref.step(x0.copy(), b)) < tol
+    d = dev(np.zeros(ref.N), prec)
+    sm.precond_vmult(d, dev(b, prec))
+    assert rel_err(host(d), ref.precond_vmult(b)) < tol
+
+
 @pytest.mark.parametrize("vec_prec", [0, 1])
 def test_bf16_mfma_patch_solves(vec_prec):
     """BF16 products with FP32 accumulation in the FD contractions: one MVP
