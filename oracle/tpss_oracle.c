@@ -549,7 +549,7 @@ or_ctx *or_create(int dim, int p, int n, double ip, int variant, int coloring,
 }
 
 /*
- * Linear elasticity (C5 of the reference apps): SIPG DG Q_p vector field with
+ * Linear elasticity: SIPG DG Q_p vector field with
  * dim components on the Cartesian unit cube, Lame coefficients mu, lambda
  * (apps/linelasticity_integrator.h:31-124 for the operator; block-diagonal
  * local solvers :604-647).  Vectors are component-blocked like the

@@ -25,9 +25,10 @@ def make_fit(dim, p, n, variant, amp, seed=3, sweeps=4, ip=4.0, precision=0):
 
 
 @pytest.mark.parametrize("n,variant", [(3, 0), (3, 1), (4, 0)])
-def test_fit_parity_q4_configuration_3(n, variant):
-    """Configuration 3's local solver: 3D DG Q4 (k = 10, 1000 DoFs per patch), interior
-    vertices moved by 0.15 h, rank-2 separable fit of the exact local matrices; the
+def test_fit_parity_q4_vertices_moved_by_0_15h(n, variant):
+    """The fitted local solver of the 48^3 Q4 workload: 3D DG Q4 (k = 10, 1000 DoFs per
+    patch), interior vertices moved by 0.15 h, rank-2 separable fit of the exact local
+    matrices; the
     additive smoother with damping 0.7 and the multiplicative one, FP64 and FP32 patch
     solves (the FP32 smoother keeps the FP64 setup)."""
     t = _tpss()
