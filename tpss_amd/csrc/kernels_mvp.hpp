@@ -176,6 +176,9 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   CT w;
   if constexpr(BATCH)
   {
+    // a starting wave issues its loads ahead of the other wave's arithmetic on
+    // the SIMD (raised priority until the last load of the batch is issued)
+    __builtin_amdgcn_s_setprio(3);
     const int jw = min(j0, patch_end - 1);  // wave-uniform
     const uint32_t pv = vtx[jw];
 #pragma unroll
@@ -275,6 +278,7 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
         load_row<NP>(x + (c_lo + cx + sy) * NDC + NP * NP * lz + NP * ly, ry[cx]);
         load_row<NP>(x + (c_lo + cx + sz) * NDC + NP * ly + NP * NP * lz, rz[cx]);
       }
+      __builtin_amdgcn_s_setprio(0);
       // z_0 = F x_out along x, z_1 / z_2 = the y- / z-neighbour planes contracted
       // along the face normal, from the rows loaded above.  The contractions run
       // for absent neighbours too and are discarded by selection: a branch would
