@@ -204,11 +204,11 @@ void fit_local_solvers(const tpss_level *lv, LocalSolvers *ls, int sweeps, const
 // other two, too far for the 4 MB L2 of an XCD, and the hardware deals
 // consecutive blocks to the 8 XCDs in turn.  Here the (x, y) extent of the color
 // is cut into 4 x 2 column tiles, one per XCD; a tile's patches go z-slab by
-// z-slab (lexicographic inside a slab), and the tiles' lists are interleaved in
-// chunks of `ppb` patches (one block), so that block i runs on XCD i mod 8 and
+// z-slab (lexicographic inside a slab), and the tiles' lists are interleaved
+// patch by patch (one patch per block), so that block i runs on XCD i mod 8 and
 // the three readers of a cell run on one XCD a slab (about 64 blocks) apart.
 // Returns the positions [begin, end) in launch order.
-std::vector<int> xcd_launch_order(const LocalSolvers &ls, int begin, int end, int ppb)
+std::vector<int> xcd_launch_order(const LocalSolvers &ls, int begin, int end)
 {
   constexpr int kXcd = 8, kTx = 4, kTy = 2;
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
@@ -237,9 +237,9 @@ std::vector<int> xcd_launch_order(const LocalSolvers &ls, int begin, int end, in
     });
   std::vector<int> order;
   order.reserve(size_t(end - begin));
-  for(size_t at = 0; order.size() < size_t(end - begin); at += size_t(ppb))
+  for(size_t at = 0; order.size() < size_t(end - begin); ++at)
     for(const auto &t : tiles)
-      for(size_t q = at; q < std::min(t.size(), at + size_t(ppb)); ++q) order.push_back(t[q]);
+      if(at < t.size()) order.push_back(t[at]);
   return order;
 }
 
@@ -399,8 +399,7 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv,
     std::vector<uint32_t> vtx_l(static_cast<size_t>(P)), fac_l(static_cast<size_t>(P) * 3);
     for(int c = 0; c < ls->n_colors; ++c)
     {
-      const std::vector<int> order =
-        xcd_launch_order(*ls, ls->color_ptr[c], ls->color_ptr[c + 1], 1);
+      const std::vector<int> order = xcd_launch_order(*ls, ls->color_ptr[c], ls->color_ptr[c + 1]);
       for(size_t q = 0; q < order.size(); ++q)
       {
         const size_t at = size_t(ls->color_ptr[c]) + q, j = size_t(order[q]);
