@@ -65,6 +65,8 @@ def parse():
                          "capped by OMP_NUM_THREADS)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the all-cores CPU run")
     ap.add_argument("--cpu-steps-1core", type=int, default=1, help="steps of the 1-core CPU run")
+    ap.add_argument("--no-cpu-gmg", action="store_true",
+                    help="skip the CPU GMG-CG solve of the baseline (about 25 s on 16 cores)")
     return ap.parse_args()
 
 
@@ -315,6 +317,19 @@ def cpu_baseline(n, p, args):
                       "median_step_seconds": med, "min_step_seconds": float(min(times)),
                       "patch_dofs_per_s": sm.P * sm.ndp / med}
         del sm
+    if not args.no_cpu_gmg:
+        # the same GMG-CG to 1e-8 as the GPU's FP64 path (6 levels, 2^3 coarse cells,
+        # MVP pre- and post-smoothing with the full residual per color), all cores
+        t0 = time.perf_counter()
+        g = o.GMG(3, p, 2, int(np.log2(n // 2)) + 1, threads=threads)
+        setup = time.perf_counter() - t0
+        b = o.splitmix_uniform(2, g.N)
+        t0 = time.perf_counter()
+        _, it, hist = g.cg(b, rel_tol=1e-8)
+        out["gmg_cg"] = {"threads": threads, "setup_seconds": setup,
+                         "seconds": time.perf_counter() - t0, "iterations": int(it),
+                         "final_rel_residual": float(hist[-1] / hist[0])}
+        del g
     out["host"] = {"usable_cpus": usable, "host_cpus": total, "model": model}
     return out
 
@@ -453,6 +468,12 @@ def main():
                          f"20 steps to keep the run within minutes",
                "one_core": c1["patch_dofs_per_s"], "runs": cb,
                "host": f"{hst['model']}, {hst['usable_cpus']} usable of {hst['host_cpus']} CPUs"}
+        if "gmg_cg" in cb:
+            gc = cb["gmg_cg"]
+            cpu["gmg_cg_seconds"] = gc["seconds"]
+            cpu["sample"] += (f"; GMG-CG to 1e-8 (FP64, same hierarchy and smoother): "
+                              f"{gc['iterations']} iterations in {gc['seconds']:.2f} s on "
+                              f"{gc['threads']} threads")
 
     main_v = results["f64"]
     dom = main_v["dominant"]

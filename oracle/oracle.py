@@ -95,6 +95,7 @@ def _load():
         "or_gmg_destroy": (None, [ctypes.c_void_p]),
         "or_gmg_level": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int]),
         "or_gmg_smoother": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+        "or_gmg_set_threads": (None, [ctypes.c_void_p, ctypes.c_int]),
         "or_gmg_vcycle": (None, [ctypes.c_void_p, _D, _D]),
         "or_gmg_cg": (ctypes.c_int, [ctypes.c_void_p, _D, _D, ctypes.c_double, ctypes.c_int, _D]),
         "or_ksvd": (ctypes.c_int, [ctypes.c_int] * 4 + [_D, ctypes.c_int, _D, _D, _D]),
@@ -363,7 +364,7 @@ class GMG:
 
     def __init__(self, dim, p, n_coarse, n_levels, ip=1.0, variant=MULTIPLICATIVE,
                  coloring=COLOR_REDBLACK, relaxation=1.0, n_steps=1, coarse_tol=1e-12,
-                 vertices=None, elasticity=None):
+                 vertices=None, elasticity=None, threads=1):
         vp = None
         if vertices is not None:
             self._v = np.ascontiguousarray(vertices, dtype=np.float64)
@@ -379,6 +380,8 @@ class GMG:
                                                   float(lam), variant, coloring, float(relaxation),
                                                   n_steps, float(coarse_tol))
             ncomp = dim
+        if threads > 1:
+            lib.or_gmg_set_threads(self.h, int(threads))
         self.N = ncomp * (n_coarse << (n_levels - 1)) ** dim * (p + 1) ** dim
 
     def __del__(self):

@@ -2083,6 +2083,17 @@ void or_gmg_destroy(or_gmg *g)
 or_ctx *or_gmg_level(or_gmg *g, int level) { return g->pre[level]; }
 or_ctx *or_gmg_smoother(or_gmg *g, int level, int post) { return post ? g->post[level] : g->pre[level]; }
 
+/* OpenMP threads of every level's operator and patch loops (the CG's vector
+ * operations stay serial) */
+void or_gmg_set_threads(or_gmg *g, int n_threads)
+{
+  for(int l = 0; l < g->n_levels; ++l)
+  {
+    or_set_threads(g->pre[l], n_threads);
+    or_set_threads(g->post[l], n_threads);
+  }
+}
+
 static double dot(long n, const double *a, const double *b)
 {
   double s = 0.0;

@@ -108,6 +108,7 @@ or_gmg *or_gmg_create_elasticity(int dim, int p, int n_coarse, int n_levels, dou
 void or_gmg_destroy(or_gmg *g);
 or_ctx *or_gmg_level(or_gmg *g, int level);
 or_ctx *or_gmg_smoother(or_gmg *g, int level, int post);
+void or_gmg_set_threads(or_gmg *g, int n_threads);
 void or_gmg_vcycle(or_gmg *g, double *dst, const double *src);
 int or_gmg_cg(or_gmg *g, double *x, const double *b, double rel_tol, int max_it,
               double *res_hist);

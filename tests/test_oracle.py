@@ -347,6 +347,17 @@ def test_gmg_iteration_bounds(dim, p, levels, variant, bound):
     assert fractional_steps(hist, 1e-6) < bound, (it, hist)
 
 
+def test_gmg_threads_do_not_change_the_result():
+    """The threaded oracle GMG (the CPU baseline's solve) computes bit for bit what the
+    serial one does: the threads split the patches of a color and the cells of an
+    operator application, whose results do not depend on the split."""
+    b = o.splitmix_uniform(2, o.GMG(3, 2, 2, 3).N)
+    x1, it1, h1 = o.GMG(3, 2, 2, 3).cg(b, rel_tol=1e-8)
+    x4, it4, h4 = o.GMG(3, 2, 2, 3, threads=4).cg(b, rel_tol=1e-8)
+    assert it1 == it4 > 0
+    assert np.array_equal(x1, x4) and np.array_equal(h1, h4)
+
+
 def fractional_steps(hist, tol):
     """compute_fractional_steps, apps/postprocess.h:40-60."""
     n = len(hist) - 1
