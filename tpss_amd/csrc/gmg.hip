@@ -8,6 +8,7 @@
 #include <cmath>
 #include <functional>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -105,7 +106,8 @@ int pcg(tpss_gmg *g, tpss_level *A, double *x, const double *b, double tol, int 
         double *p, double *Ap, double *sc, float *r_low)
 {
   const int64_t N = A->N;
-  // sc: [rz, pAp, rr, rz_new, rz_old]
+  // sc: [rz, pAp, rr, rz'] -- r.z of the current and of the next iteration in
+  // sc[0] and sc[3] by turns (rz / rz_next swap on the host, no copies)
   fill_zero(x, TPSS_F64, N, s);
   TPSS_HIP(hipMemcpyAsync(r, b, N * sizeof(double), hipMemcpyDeviceToDevice, s));
   dot64(g, r, r, N, sc + 2, s);
@@ -121,7 +123,8 @@ int pcg(tpss_gmg *g, tpss_level *A, double *x, const double *b, double tol, int 
       dot64(g, r, z, N, rz, s);
     }
   };
-  apply_precond(sc + 0, false);
+  double *rz = sc + 0, *rz_next = sc + 3;
+  apply_precond(rz, false);
   TPSS_HIP(hipMemcpyAsync(p, z, N * sizeof(double), hipMemcpyDeviceToDevice, s));
   const unsigned grid = blocks_for(N, 256);
   const int nb = DotWorkspace::blocks(N);
@@ -130,22 +133,21 @@ int pcg(tpss_gmg *g, tpss_level *A, double *x, const double *b, double tol, int 
     operator_apply_any(A, Ap, p, nullptr, nullptr, 0, s);
     dot64(g, p, Ap, N, sc + 1, s);
     if(r_low)
-      hipLaunchKernelGGL(k_cg_update_dot<true>, dim3(nb), dim3(256), 0, s, x, r, p, Ap, N, sc,
-                         g->dots.partial.ptr, r_low);
+      hipLaunchKernelGGL(k_cg_update_dot<true>, dim3(nb), dim3(256), 0, s, x, r, p, Ap, N, rz,
+                         sc + 1, g->dots.partial.ptr, r_low);
     else
-      hipLaunchKernelGGL(k_cg_update_dot<false>, dim3(nb), dim3(256), 0, s, x, r, p, Ap, N, sc,
-                         g->dots.partial.ptr, nullptr);
+      hipLaunchKernelGGL(k_cg_update_dot<false>, dim3(nb), dim3(256), 0, s, x, r, p, Ap, N, rz,
+                         sc + 1, g->dots.partial.ptr, nullptr);
     TPSS_HIP(hipGetLastError());
     g->dots.finish(nb, sc + 2, s);
     const double rn = std::sqrt(read_scalar(sc + 2, s));
     if(hist) hist->push_back(rn);
     if(!std::isfinite(rn)) throw Error(TPSS_E_NONFINITE, "non-finite residual in CG");
     if(rn <= tol * r0) return it;
-    TPSS_HIP(hipMemcpyAsync(sc + 4, sc + 0, sizeof(double), hipMemcpyDeviceToDevice, s));
-    apply_precond(sc + 3, r_low != nullptr);
-    hipLaunchKernelGGL(k_cg_direction, dim3(grid), dim3(256), 0, s, p, z, N, sc + 3);
+    apply_precond(rz_next, r_low != nullptr);
+    hipLaunchKernelGGL(k_cg_direction, dim3(grid), dim3(256), 0, s, p, z, N, rz_next, rz);
     TPSS_HIP(hipGetLastError());
-    TPSS_HIP(hipMemcpyAsync(sc + 0, sc + 3, sizeof(double), hipMemcpyDeviceToDevice, s));
+    std::swap(rz, rz_next);
   }
   return -1;
 }
@@ -376,7 +378,7 @@ int tpss_gmg_create(const tpss_gmg_desc *desc, tpss_gmg **out)
       g->z.resize(N);
       g->p.resize(N);
       g->Ap.resize(N);
-      g->scal.resize(16);  // outer CG [0, 5), coarse CG [8, 13)
+      g->scal.resize(16);  // outer CG [0, 4), coarse CG [8, 12)
     }
     catch(...)
     {

@@ -1363,18 +1363,19 @@ __device__ __forceinline__ double block_sum_256(double v, double *sh)
   return sh[0];
 }
 
-// CG update with device scalars, alpha = s[0]/s[1]:  x += alpha p,
+// CG update with device scalars, alpha = rz / pAp:  x += alpha p,
 // r -= alpha Ap, block partial sums of the updated r.r (the grid-stride
 // order of k_dot_partial, so the sums match dot(r, r) bit for bit) and, if
 // r_low is given, the updated r in FP32 (the right-hand side of an FP32 V-cycle)
 template<bool LOW>
 __global__ void __launch_bounds__(256)
 k_cg_update_dot(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
-                const double *__restrict__ Ap, const int64_t n, const double *__restrict__ s,
-                double *__restrict__ partial, float *__restrict__ r_low)
+                const double *__restrict__ Ap, const int64_t n, const double *__restrict__ rz,
+                const double *__restrict__ pAp, double *__restrict__ partial,
+                float *__restrict__ r_low)
 {
   __shared__ double sh[256];
-  const double alpha = s[0] / s[1];
+  const double alpha = *rz / *pAp;
   double acc = 0.0;
   for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
       i += int64_t(gridDim.x) * blockDim.x)
@@ -1408,12 +1409,13 @@ k_convert_dot(double *__restrict__ z, const T *__restrict__ v, const double *__r
   if(threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// p = z + (s[0]/s[1]) p
+// p = z + (rz_new / rz_old) p
 static __global__ void k_cg_direction(double *__restrict__ p, const double *__restrict__ z,
-                               const int64_t n, const double *__restrict__ s)
+                                      const int64_t n, const double *__restrict__ rz_new,
+                                      const double *__restrict__ rz_old)
 {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const double beta = s[0] / s[1];
+  const double beta = *rz_new / *rz_old;
   if(i < n) p[i] = z[i] + beta * p[i];
 }
 
