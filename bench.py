@@ -214,11 +214,13 @@ def elasticity_section(t, args, n=64, p=2, mu=1.0, lam=10.0, ip=10.0):
     out = {"workload": f"3D SIPG DG Q{p} linear elasticity {n}^3 cells, mu {mu}, lambda {lam}, "
                        f"ip {ip}, block-diagonal vertex patches"}
     steps = max(2, args.steps // 4)
-    for name, variant, relax, prec in (("mvp_f32", t.MULTIPLICATIVE, 1.0, t.F32),
-                                       ("avp_f32", t.ADDITIVE, 0.15, t.F32),
-                                       ("mvp_f64", t.MULTIPLICATIVE, 1.0, t.F64)):
+    for name, variant, relax, prec, lprec in (
+            ("mvp_f32", t.MULTIPLICATIVE, 1.0, t.F32, t.F32),
+            ("avp_f32", t.ADDITIVE, 0.15, t.F32, t.F32),
+            ("mvp_f64", t.MULTIPLICATIVE, 1.0, t.F64, t.F64),
+            ("mvp_bf16_mfma", t.MULTIPLICATIVE, 1.0, t.F32, t.BF16)):
         lv = t.Level(3, p, n, ip_factor=ip, precision=prec, elasticity=(mu, lam))
-        sm = t.SchwarzSmoother(lv, variant=variant, relaxation=relax, local_precision=prec)
+        sm = t.SchwarzSmoother(lv, variant=variant, relaxation=relax, local_precision=lprec)
         sec, sections = time_smoother(t, lv, sm, steps, 1, prec)
         apply_t, apply_n = sections["[SchwarzPrecond] Apply inverses:"]
         res_t, res_n = sections["[SchwarzPrecond] Update residuals:"]
@@ -236,11 +238,14 @@ def elasticity_section(t, args, n=64, p=2, mu=1.0, lam=10.0, ip=10.0):
         from tpss_amd import learned_damping as ld
         mlp = ld.load_elasticity()
         out["learned_damping"] = float(mlp.forward(np.ones((1, 24), dtype=np.float32))[0])
-        for name, mp, net in (("gmg_cg_f64", t.F64, None), ("gmg_cg_mixed_f32", t.F32, None),
-                              ("gmg_cg_f64_learned", t.F64, mlp),
-                              ("gmg_cg_mixed_f32_learned", t.F32, mlp)):
+        for name, mp, net, lp in (("gmg_cg_f64", t.F64, None, None),
+                                  ("gmg_cg_mixed_f32", t.F32, None, None),
+                                  ("gmg_cg_mixed_bf16_mfma", t.F32, None, t.BF16),
+                                  ("gmg_cg_f64_learned", t.F64, mlp, None),
+                                  ("gmg_cg_mixed_f32_learned", t.F32, mlp, None)):
             g = t.GMGSolver(3, p, 2, int(np.log2(n // 2)) + 1, ip_factor=ip, mg_precision=mp,
-                            variant=t.MULTIPLICATIVE, relaxation=1.0, elasticity=(mu, lam))
+                            variant=t.MULTIPLICATIVE, relaxation=1.0, elasticity=(mu, lam),
+                            local_precision=lp)
             if net is not None:
                 ld.apply_to_gmg(g, net)
             b = t.DeviceVector(g.n_dofs)

@@ -96,7 +96,9 @@ typedef struct
   int symmetrized;       /* colors 0..C-1..0 */
   double damping_factor; /* Richardson omega_R */
   int n_steps;           /* Richardson steps per step() */
-  int local_precision;   /* tpss_precision of the patch solves */
+  int local_precision;   /* tpss_precision of the patch solves; TPSS_BF16 (BF16-MFMA
+                          * products, FP32 sums): multiplicative, red-black, Cartesian 3D
+                          * Q3 Poisson or 3D Q2 elasticity */
   int local_solver;      /* tpss_local_solver (0 = surrogate lengths) */
   int fit_sweeps;        /* ALS sweeps of TPSS_LOCAL_FIT (0 = 4) */
   int ras;               /* restricted additive Schwarz (additive variant, Poisson):

@@ -92,11 +92,48 @@ def test_elasticity_fp32_patch_solves(variant):
     assert rel_err(host(d), ref.precond_vmult(src)) < 2e-5
 
 
-def test_elasticity_bf16_is_rejected():
+@pytest.mark.parametrize("vec_prec,rev", [(1, False), (0, True)])
+def test_elasticity_bf16_mfma_patch_solves(vec_prec, rev):
+    """3D Q2 elasticity, block-diagonal multiplicative smoother with BF16-MFMA patch
+    solves (each 6^3 component block zero-padded into the 8^3 layout of the matrix-core
+    solve, FP32 accumulation): one sweep within 3e-2 (max norm) of the FP64 oracle, and
+    really computed in reduced precision."""
+    t = _tpss()
+    lv, sm, ref = make(3, 2, 5, t.MULTIPLICATIVE, precision=vec_prec, local_precision=t.BF16,
+                       reverse=rev)
+    src = o.splitmix_uniform(7, ref.N)
+    d = dev(np.zeros(ref.N), vec_prec)
+    sm.precond_vmult(d, dev(src, vec_prec))
+    err = rel_err(host(d), ref.precond_vmult(src))
+    assert 1e-7 < err < 3e-2, err
+    # the Richardson step runs the same sweep on the residual
+    x0 = o.splitmix_uniform(8, ref.N)
+    x = dev(x0, vec_prec)
+    sm.step(x, dev(src, vec_prec))
+    err = rel_err(host(x), ref.step(x0.copy(), src))
+    assert err < 3e-2, err
+
+
+def test_elasticity_bf16_outside_its_scope_is_rejected():
+    """BF16-MFMA solves of elasticity blocks: 3D Q2, multiplicative, red-black only."""
     t = _tpss()
     lv = t.Level(3, 2, 4, ip_factor=10.0, elasticity=(MU, LAM), precision=t.F32)
     with pytest.raises(t.TPSSError):
-        t.SchwarzSmoother(lv, variant=t.MULTIPLICATIVE, local_precision=t.BF16)
+        t.SchwarzSmoother(lv, variant=t.ADDITIVE, local_precision=t.BF16)
+    lv3 = t.Level(3, 3, 3, ip_factor=10.0, elasticity=(MU, LAM), precision=t.F32)
+    with pytest.raises(t.TPSSError):
+        t.SchwarzSmoother(lv3, variant=t.MULTIPLICATIVE, local_precision=t.BF16)
+
+
+def test_elasticity_bf16_gmg_reaches_tolerance():
+    """The BF16-MFMA smoother inside the FP32 V-cycle of the FP64 CG (3D Q2, 16^3)."""
+    t = _tpss()
+    g = t.GMGSolver(3, 2, 2, 4, ip_factor=10.0, mg_precision=t.F32, local_precision=t.BF16,
+                    elasticity=(MU, LAM))
+    b = o.splitmix_uniform(2, g.n_dofs)
+    x = dev(np.zeros(g.n_dofs))
+    info, hist = g.solve(x, dev(b), rel_tol=1e-8, max_iter=40)
+    assert hist[-1] <= 1e-8 * hist[0] and info.iterations <= 20, (info.iterations, hist[-1] / hist[0])
 
 
 @pytest.mark.parametrize("dim,p,levels,variant", [(2, 2, 3, 1), (2, 3, 3, 0), (3, 2, 2, 1)])

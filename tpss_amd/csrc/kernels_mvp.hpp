@@ -596,4 +596,66 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   store_row<NP>(x + (c_lo + 1) * NDC + lyz, o + NP);
 }
 
+// BF16-MFMA patch solves of the block-diagonal elasticity smoother (3D Q2, k = 6),
+// residual-update form (MODE 0 of k_patch_fd): dst += omega_j R_j^T A_jc^{-1} R_j r
+// for one component block c, two patches per 128-thread block.  The 6 x 6 x 6
+// block is embedded in the 8 x 8 x 8 layout of fd_mfma_pair and zero-padded: the
+// padded rows and columns of the eigenvector matrices are zero, so padded entries
+// stay zero through every contraction, and the padded eigenvalues are one (no
+// division by zero).  src, dst, fac: the component's block and factor rows.
+template<typename VT>
+__global__ void __launch_bounds__(128)
+k_patch_fd_bf16_q2(const VT *__restrict__ src, VT *__restrict__ dst,
+                   const uint32_t *__restrict__ vtx, const uint32_t *__restrict__ fac,
+                   const float *__restrict__ S_tab, const float *__restrict__ lam_tab,
+                   const float *__restrict__ omega, const float relaxation,
+                   const int patch_begin, const int patch_end, const int n)
+{
+  constexpr int K = 6, NP = 3, NDC = 27;
+  __shared__ float B2[2][512];
+  __shared__ float Ss[2][3][64];
+  __shared__ float ls[2][3][8];
+  __shared__ float w2[2];
+  const int j0 = patch_begin + 2 * int(blockIdx.x);
+  for(int idx = threadIdx.x; idx < 2 * 3 * 64; idx += 128)
+  {
+    const int q = idx / 192, r = idx % 192, d = r / 64, e = r % 64, a = e / 8, i = e % 8;
+    const int j = j0 + q;
+    float v = 0.f;
+    if(j < patch_end && a < K && i < K) v = S_tab[size_t(fac[size_t(j) * 3 + d]) * K * K + a * K + i];
+    Ss[q][d][e] = v;
+  }
+  for(int idx = threadIdx.x; idx < 2 * 3 * 8; idx += 128)
+  {
+    const int q = idx / 24, r = idx % 24, d = r / 8, i = r % 8;
+    const int j = j0 + q;
+    ls[q][d][i] = (j < patch_end && i < K) ? lam_tab[size_t(fac[size_t(j) * 3 + d]) * K + i] : 1.f;
+  }
+  if(threadIdx.x < 2)
+  {
+    const int j = j0 + int(threadIdx.x);
+    w2[threadIdx.x] = j < patch_end ? (omega ? omega[j] : relaxation) : 0.f;
+  }
+  // global index of padded entry e (x fastest) of patch slot q, or -1
+  auto entry = [&](int q, int e) -> int64_t {
+    const int j = j0 + q, x = e % 8, y = (e / 8) % 8, z = e / 64;
+    if(j >= patch_end || x >= K || y >= K || z >= K) return -1;
+    int v[3];
+    unpack_vertex(vtx[j], v);
+    return patch_cell<3, NP>(v, n, x, y, z) * NDC + patch_local<3, NP>(x, y, z);
+  };
+  for(int idx = threadIdx.x; idx < 2 * 512; idx += 128)
+  {
+    const int64_t g = entry(idx / 512, idx % 512);
+    B2[idx / 512][idx % 512] = g >= 0 ? float(src[g]) : 0.f;
+  }
+  __syncthreads();
+  fd_mfma_pair(B2, Ss, ls, w2);  // ends with a barrier: B2 holds the corrections
+  for(int idx = threadIdx.x; idx < 2 * 512; idx += 128)
+  {
+    const int64_t g = entry(idx / 512, idx % 512);
+    if(g >= 0) dst[g] += VT(B2[idx / 512][idx % 512]);
+  }
+}
+
 }  // namespace tpss

@@ -751,16 +751,69 @@ void precond_vmult_t(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, b
 
 // BF16-MFMA patch solves (FP32 accumulation) for the multiplicative variant on
 // Cartesian meshes with red-black colors, 3D Q3; vectors FP32 or FP64
+// BF16-MFMA patch solves of the elasticity smoother (3D Q2, block diagonal):
+// the residual-update color sweep, the residual in the vector precision, the
+// component blocks' solves on the matrix cores
+template<typename VT>
+void precond_vmult_bf16_elasticity(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s,
+                                   bool accumulate, double alpha, const std::vector<int> &seq)
+{
+  LocalSolvers &ls = *sm->ls;
+  tpss_level *lv = sm->level;
+  SectionTimer step_timer(sm, 4, s);
+  VT *out = dst;
+  if(accumulate)
+  {
+    sm->tmp2.resize(size_t(lv->N) * sizeof(VT));
+    out = (VT *)sm->tmp2.ptr;
+  }
+  VT *res = (VT *)sm->scratch;
+  if(!res)
+  {
+    sm->res.resize(size_t(lv->N) * sizeof(VT));
+    res = (VT *)sm->res.ptr;
+  }
+  const float *om = sm->use_omega ? sm->d_omega.ptr : nullptr;
+  fill_zero(out, lv->vec_prec, lv->N, s);
+  for(size_t k = 0; k < seq.size(); ++k)
+  {
+    const int c = seq[k];
+    const int begin = ls.color_ptr[c], end = ls.color_ptr[c + 1];
+    const VT *r = src;
+    if(k > 0)
+    {
+      SectionTimer t(sm, 0, s);
+      operator_apply_any(lv, res, out, src, ls.color_cells[c]->ptr, ls.color_ncells[c], s);
+      r = res;
+    }
+    if(end <= begin) continue;
+    SectionTimer t(sm, 1, s);
+    for(int comp = 0; comp < ls.ncomp; ++comp)
+      hipLaunchKernelGGL((k_patch_fd_bf16_q2<VT>), dim3(unsigned((end - begin + 1) / 2)), dim3(128),
+                         0, s, r + comp * lv->Ns, out + comp * lv->Ns, ls.d_vtx.ptr,
+                         ls.d_fac.ptr + size_t(comp) * ls.P * 3, ls.S32.ptr, ls.lam32.ptr, om,
+                         float(sm->desc.relaxation), begin, end, ls.n);
+    TPSS_HIP(hipGetLastError());
+  }
+  if(accumulate) axpby(dst, out, lv->vec_prec, lv->N, alpha, 1.0, s);
+}
+
 template<typename VT>
 void precond_vmult_bf16(tpss_smoother *sm, VT *dst, const VT *src, hipStream_t s, bool accumulate,
                         double alpha, const std::vector<int> &seq)
 {
   LocalSolvers &ls = *sm->ls;
   const tpss_level *lv = sm->level;
+  if(sm->desc.variant == TPSS_MULTIPLICATIVE && lv->ncomp == 3 && ls.coloring == TPSS_COLOR_REDBLACK &&
+     ls.dim == 3 && ls.K == 6)
+  {
+    precond_vmult_bf16_elasticity(sm, dst, src, s, accumulate, alpha, seq);
+    return;
+  }
   if(sm->desc.variant != TPSS_MULTIPLICATIVE || lv->distorted || lv->ncomp > 1 ||
      ls.coloring != TPSS_COLOR_REDBLACK || ls.dim != 3 || ls.K != 8)
-    throw Error(TPSS_E_UNSUPPORTED,
-                "BF16-MFMA patch solves: multiplicative, red-black, Cartesian 3D Q3 only");
+    throw Error(TPSS_E_UNSUPPORTED, "BF16-MFMA patch solves: multiplicative, red-black, Cartesian "
+                                    "3D Q3 (Poisson) or 3D Q2 (elasticity) only");
   SectionTimer step_timer(sm, 4, s);
   VT *out = dst;
   if(accumulate)
@@ -887,7 +940,7 @@ size_t precond_scratch_bytes(const tpss_smoother *sm)
   const int lp = sm->desc.local_precision;
   const size_t ct = lp == TPSS_F64 ? 8 : 4;
   if(sm->desc.variant == TPSS_ADDITIVE) return size_t(lv->Ns) * (size_t(1) << lv->dim) * ct;
-  if(lp == TPSS_BF16) return 0;
+  if(lp == TPSS_BF16) return lv->ncomp > 1 ? size_t(lv->N) * elem_size(lv->vec_prec) : 0;
   if(!lv->distorted && lv->ncomp == 1 && ls.coloring == TPSS_COLOR_REDBLACK && sm->fused_mvp)
     return 0;
   return size_t(lv->N) * elem_size(lv->vec_prec);
@@ -981,8 +1034,11 @@ int tpss_smoother_create(tpss_level *level, const tpss_smoother_desc *desc, tpss
     validate_desc(desc);
     if(level->vec_prec == TPSS_F32 && desc->local_precision == TPSS_F64)
       throw Error(TPSS_E_UNSUPPORTED, "local precision F64 needs F64 level vectors");
-    if(level->ncomp > 1 && desc->local_precision == TPSS_BF16)
-      throw Error(TPSS_E_UNSUPPORTED, "linear elasticity: local precision F64 or F32");
+    if(level->ncomp > 1 && desc->local_precision == TPSS_BF16 &&
+       !(level->dim == 3 && level->p == 2 && desc->variant == TPSS_MULTIPLICATIVE &&
+         desc->coloring == TPSS_COLOR_REDBLACK))
+      throw Error(TPSS_E_UNSUPPORTED, "linear elasticity with BF16-MFMA patch solves: 3D Q2, "
+                                      "multiplicative, red-black colors only");
     if(level->ncomp > 1 && desc->local_solver == TPSS_LOCAL_FIT)
       throw Error(TPSS_E_UNSUPPORTED, "fitted local solvers: Poisson levels only");
     // dof_info.templates.h:208: "TODO more than scalar elements"
