@@ -168,8 +168,9 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
   // neighbour.  Each lane loads one row of the y- and z-neighbour planes and
   // takes the other three rows of a plane from the lanes that loaded them: the
   // lanes sharing a y-plane form a quad (DPP), those sharing a z-plane sit 8
-  // lanes apart (ds_bpermute).  The patch tables are staged unconditionally
-  // (the full matrices are unused by the even-odd products).
+  // lanes apart (their rows pass through LDS: 4 writes and 16 reads of 16 bytes
+  // per lane instead of 64 ds_bpermute).  The patch tables are staged
+  // unconditionally (the full matrices are unused by the even-odd products).
   constexpr bool BATCH = WAVE_PATCH && DIM == 3 && K == 8 && !std::is_same<CT, float>::value;
   int v[3] = {1, 1, 1};
   uint32_t fw[3] = {0, 0, 0};  // factor rows of the wave's patch (BATCH)
@@ -330,9 +331,16 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
           }
       }
       {
-        // the lanes of line (ty, NP bz + kk), kk = 0..3, hold the rows kk
+        // the lanes of line (ty, NP bz + kk), kk = 0..3, hold the rows kk: exchanged
+        // through LDS (A2 is free until the coupling stage writes T2 there)
         const CT *F = bz == 0 ? tab.Fpm : tab.Fmp;
         CT z2[K];
+        CT *RZ = A2[0];
+#pragma unroll
+        for(int cx = 0; cx < 2; ++cx)
+#pragma unroll
+          for(int lx = 0; lx < NP; ++lx) RZ[(t * 2 + cx) * NP + lx] = rz[cx][lx];
+        __syncthreads();
 #pragma unroll
         for(int cx = 0; cx < 2; ++cx)
         {
@@ -340,7 +348,7 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
 #pragma unroll
           for(int kk = 0; kk < NP; ++kk)
 #pragma unroll
-            for(int lx = 0; lx < NP; ++lx) xn[lx + NP * kk] = lane_read(rz[cx][lx], lane0 | (kk << 3));
+            for(int lx = 0; lx < NP; ++lx) xn[lx + NP * kk] = RZ[((lane0 | (kk << 3)) * 2 + cx) * NP + lx];
 #pragma unroll
           for(int lx = 0; lx < NP; ++lx)
           {
@@ -465,6 +473,7 @@ k_patch_mvp_fused(const VT *__restrict__ b, VT *__restrict__ x, const OpTab<CT, 
         }
       }
     }
+    if constexpr(BATCH) __syncthreads();  // the z-plane rows in A2 are read
     // cross-line mass contractions through LDS
     CT *Z = A1[plc];
     CT *T2 = A2[plc];
