@@ -575,7 +575,8 @@ k_sipg_lines(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
 // six face neighbours (absent neighbours read the cell itself and are zeroed by
 // selection) -- and takes the other rows of a neighbour plane from the lanes
 // that loaded them: those of a y-plane are the lanes of its quad (DPP
-// broadcast), those of a z-plane sit 4 lanes apart (ds_bpermute).  The 1D
+// broadcast), those of a z-plane sit 4 lanes apart (staged in LDS, laid out
+// as the neighbour cell).  The 1D
 // tables indexed by a lane-varying row (mass, face coupling, boundary-dependent
 // cell matrices) are read from LDS.  Same arithmetic, in the same order, as
 // k_sipg_lines.
@@ -681,21 +682,30 @@ k_sipg_lines_q3(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restri
       for(int k = 0; k < NP; ++k) s += D[lz * NP + k] * xs[cs][i + NP * ly + NL * k];
       z2[i] = s;
     }
-    const int base = int(threadIdx.x & 63) & ~(NL - 1);  // the cell's first lane
+    // the z-neighbours' rows of every lane through LDS (zs / ws are free until
+    // z_0 and M_x z_2 are staged there): each neighbour cell laid out like xs
+#pragma unroll
+    for(int i = 0; i < NP; ++i)
+    {
+      zs[cs][lofs + i] = za[i];
+      ws[cs][lofs + i] = zb[i];
+    }
+    __syncthreads();
 #pragma unroll
     for(int side = 0; side < 2; ++side)
     {
       const CT *F = side == 0 ? tFpm : tFmp;
+      const CT *nb = side == 0 ? zs[cs] : ws[cs];
 #pragma unroll
       for(int k = 0; k < NP; ++k)
       {
-        // row (ly, k) of the neighbour: loaded by lane (ly, k) of this cell
+        // row (ly, k) of the neighbour
         const CT f = F[lz * NP + k];
-        const int src = base + ly + NP * k;
 #pragma unroll
-        for(int i = 0; i < NP; ++i) z2[i] += f * lane_read(side == 0 ? za[i] : zb[i], src);
+        for(int i = 0; i < NP; ++i) z2[i] += f * nb[i + NP * ly + NL * k];
       }
     }
+    __syncthreads();  // every lane has read the rows before zs / ws are rewritten
   }
   // x mass contractions in registers
   CT m1[NP], w2[NP];
