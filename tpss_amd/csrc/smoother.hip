@@ -539,8 +539,8 @@ void patch_pass(tpss_smoother *sm, const VT *src, VT *dst, CT *ubuf, int begin, 
 // four waves per patch shorten the pass where a color holds too few patches to
 // fill the GPU (measured per pass, FP32 / FP64: 16^3 cells, about 220 patches
 // per color, 5.1 / 6.1 us against 5.8 / 7.8 us with one wave per patch), and
-// lengthen it where the patches fill the GPU (32^3: 14.4 / 19.8 against
-// 9.6 / 14.4 us)
+// lengthen it where the patches fill the GPU (32^3, about 1860 patches per
+// color: 12.3 / 17.2 against 7.6 / 12.1 us)
 constexpr int kSplitMaxPatches = 512;
 
 template<typename VT, typename CT, int FD = 0>
