@@ -214,6 +214,20 @@ void operator_apply_f64(const tpss_level *lv, double *y, const double *x, const 
   operator_apply<double>(lv, y, x, nullptr, cells, n_list, s);
 }
 
+int operator_apply_dot_f64(const tpss_level *lv, double *y, const double *x, double *partial,
+                           int64_t cap, hipStream_t s)
+{
+  if(lv->vec_prec != TPSS_F64 || lv->ncomp != 1 || lv->distorted || lv->dim != 3 || lv->np != 4)
+    return 0;
+  const unsigned grid = blocks_for(lv->ncell, 16);
+  if(int64_t(grid) > cap) return 0;
+  const auto tab = make_optab<double, 4>(lv->op1d);
+  hipLaunchKernelGGL((k_sipg_lines_q3<double, double, false, true>), dim3(grid), dim3(256), 0, s,
+                     x, y, nullptr, tab, lv->n, lv->ncell, partial);
+  TPSS_HIP(hipGetLastError());
+  return int(grid);
+}
+
 size_t elem_size(int prec) { return prec == TPSS_F64 ? 8 : 4; }
 
 template<typename T>

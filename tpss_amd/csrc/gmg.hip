@@ -34,6 +34,9 @@ struct tpss_gmg
   // CG (FP64)
   tpss::DeviceArray<double> r, z, p, Ap, scal, cr, cz, cp, cAp;
   tpss::DotWorkspace dots;
+  // per-block parts of p.Ap formed by the operator pass of the outer CG (FP64
+  // 3D Q3 Cartesian fine levels; empty elsewhere)
+  tpss::DeviceArray<double> op_partial;
   std::vector<double> fine_vertices;
 };
 
@@ -130,8 +133,20 @@ int pcg(tpss_gmg *g, tpss_level *A, double *x, const double *b, double tol, int 
   const int nb = DotWorkspace::blocks(N);
   for(int it = 1; it <= maxit; ++it)
   {
-    operator_apply_any(A, Ap, p, nullptr, nullptr, 0, s);
-    dot64(g, p, Ap, N, sc + 1, s);
+    // Ap and p.Ap in one pass where the level has the fused form
+    const int nbo = operator_apply_dot_f64(A, Ap, p, g->op_partial.ptr,
+                                           int64_t(g->op_partial.n), s);
+    if(nbo > 0)
+    {
+      hipLaunchKernelGGL(k_dot_final_wide, dim3(1), dim3(1024), 0, s, g->op_partial.ptr, nbo,
+                         sc + 1);
+      TPSS_HIP(hipGetLastError());
+    }
+    else
+    {
+      operator_apply_any(A, Ap, p, nullptr, nullptr, 0, s);
+      dot64(g, p, Ap, N, sc + 1, s);
+    }
     if(r_low)
       hipLaunchKernelGGL(k_cg_update_dot<true>, dim3(nb), dim3(256), 0, s, x, r, p, Ap, N, rz,
                          sc + 1, g->dots.partial.ptr, r_low);
@@ -379,6 +394,11 @@ int tpss_gmg_create(const tpss_gmg_desc *desc, tpss_gmg **out)
       g->p.resize(N);
       g->Ap.resize(N);
       g->scal.resize(16);  // outer CG [0, 4), coarse CG [8, 12)
+      {
+        const tpss_level *f = g->fine64;
+        if(f->ncomp == 1 && !f->distorted && f->dim == 3 && f->np == 4)
+          g->op_partial.resize(blocks_for(f->ncell, 16));
+      }
     }
     catch(...)
     {

@@ -579,12 +579,15 @@ k_sipg_lines(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict_
 // as the neighbour cell).  The 1D
 // tables indexed by a lane-varying row (mass, face coupling, boundary-dependent
 // cell matrices) are read from LDS.  Same arithmetic, in the same order, as
-// k_sipg_lines.
-template<typename VT, typename CT, bool RESIDUAL>
+// k_sipg_lines.  DOT (operator form): also the block's part of x.y in
+// partial[blockIdx.x] (the CG's p.Ap without a second pass over p and Ap).
+template<typename VT, typename CT, bool RESIDUAL, bool DOT = false>
 __global__ void __launch_bounds__(256, 4)
 k_sipg_lines_q3(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restrict__ b,
-                const OpTab<CT, 4> tab, const int n, const int64_t ncell)
+                const OpTab<CT, 4> tab, const int n, const int64_t ncell,
+                double *__restrict__ partial = nullptr)
 {
+  static_assert(!(DOT && RESIDUAL), "the dot is formed by the operator form");
   constexpr int NP = 4, NL = 16, NDC = 64, CPB = 16;
   static_assert(CPB * NL == 256, "16 cells of 16 lines per block");
   __shared__ CT xs[CPB][NDC];  // own cells
@@ -756,6 +759,19 @@ k_sipg_lines_q3(const VT *__restrict__ x, VT *__restrict__ y, const VT *__restri
 #pragma unroll
     for(int k = 0; k < NP; ++k) s += tM[lz * NP + k] * zs[cs][i + NP * ly + NL * k];
     out[i] = s;
+  }
+  if constexpr(DOT)
+  {
+    __shared__ double wsum[4];
+    double d = 0.0;
+    if(act)
+#pragma unroll
+      for(int i = 0; i < NP; ++i) d += double(xo[i]) * double(out[i]);
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off);
+    if((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if(threadIdx.x == 0) partial[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
   }
   if(!act) return;
   if(RESIDUAL)
@@ -1345,6 +1361,37 @@ k_dot_final(const double *__restrict__ partial, const int np, double *__restrict
     __syncthreads();
   }
   if(threadIdx.x == 0) *out = sh[0];
+}
+
+// *out = the sum of np partial sums, np up to tens of thousands (one per block
+// of an operator pass): 1024 threads with four independent loads in flight each
+static __global__ void __launch_bounds__(1024)
+k_dot_final_wide(const double *__restrict__ partial, const int np, double *__restrict__ out)
+{
+  __shared__ double sh[16];
+  constexpr int T = 1024;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = threadIdx.x;
+  for(; i + 3 * T < np; i += 4 * T)
+  {
+    s0 += partial[i];
+    s1 += partial[i + T];
+    s2 += partial[i + 2 * T];
+    s3 += partial[i + 3 * T];
+  }
+  for(; i < np; i += T) s0 += partial[i];
+  double s = (s0 + s1) + (s2 + s3);
+#pragma unroll
+  for(int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if(threadIdx.x < 64)
+  {
+    double v = threadIdx.x < 16 ? sh[threadIdx.x] : 0.0;
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if(threadIdx.x == 0) *out = v;
+  }
 }
 
 // non-finite detector (grid-stride): flag[0] = 1 if any entry is NaN/Inf;

@@ -39,6 +39,11 @@ void operator_apply_any(const tpss_level *lv, void *y, const void *x, const void
 // y = A x in FP64 whatever the level's vector precision (setup of fitted local solvers)
 void operator_apply_f64(const tpss_level *lv, double *y, const double *x, const uint32_t *cells,
                         int64_t n_list, hipStream_t s);
+// y = A x and the per-block partial sums of x.y in partial[0, nb) (at most cap
+// of them) on FP64 3D Q3 Cartesian levels: returns nb, or 0 (nothing launched)
+// on other levels
+int operator_apply_dot_f64(const tpss_level *lv, double *y, const double *x, double *partial,
+                           int64_t cap, hipStream_t s);
 void fill_zero(void *x, int prec, int64_t n, hipStream_t s);
 void axpby(void *y, const void *x, int prec, int64_t n, double a, double b, hipStream_t s);
 void convert(void *dst, int dp, const void *src, int sp, int64_t n, hipStream_t s);
