@@ -203,14 +203,18 @@ void fit_local_solvers(const tpss_level *lv, LocalSolvers *ls, int sweeps, const
 // comes one plane of patches (about 500 blocks, 10 MB of traffic) after the
 // other two, too far for the 4 MB L2 of an XCD, and the hardware deals
 // consecutive blocks to the 8 XCDs in turn.  Here the (x, y) extent of the color
-// is cut into 4 x 2 column tiles, one per XCD; a tile's patches go z-slab by
-// z-slab (lexicographic inside a slab), and the tiles' lists are interleaved
-// patch by patch (one patch per block), so that block i runs on XCD i mod 8 and
-// the three readers of a cell run on one XCD a slab (about 64 blocks) apart.
+// is cut into kTx x (8 / kTx) column tiles, one per XCD; a tile's patches go
+// z-slab by z-slab (lexicographic inside a slab), and the tiles' lists are
+// interleaved patch by patch (one patch per block), so that block i runs on XCD
+// i mod 8 and the three readers of a cell run on one XCD a slab apart.  Tiles
+// measured per 64^3 Q3 step: FP64 passes 0.914 ms with 1 x 8 (full-width y
+// bands), 0.920 with 2 x 4, 0.927 with 4 x 2, 0.969 with 8 x 1; FP32 passes
+// 0.453 ms with 4 x 2 or 2 x 4, 0.459 with 1 x 8.
 // Returns the positions [begin, end) in launch order.
-std::vector<int> xcd_launch_order(const LocalSolvers &ls, int begin, int end)
+std::vector<int> xcd_launch_order(const LocalSolvers &ls, int begin, int end, int kTx)
 {
-  constexpr int kXcd = 8, kTx = 4, kTy = 2;
+  constexpr int kXcd = 8;
+  const int kTy = kXcd / kTx;
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
   auto block = [&](int j, int d) {
     const int x = ls.vtx_unpacked[size_t(j) * 3 + d] - 1;
@@ -399,7 +403,8 @@ std::shared_ptr<LocalSolvers> build_local_solvers(const tpss_level *lv,
     std::vector<uint32_t> vtx_l(static_cast<size_t>(P)), fac_l(static_cast<size_t>(P) * 3);
     for(int c = 0; c < ls->n_colors; ++c)
     {
-      const std::vector<int> order = xcd_launch_order(*ls, ls->color_ptr[c], ls->color_ptr[c + 1]);
+      const std::vector<int> order = xcd_launch_order(
+        *ls, ls->color_ptr[c], ls->color_ptr[c + 1], desc.local_precision == TPSS_F64 ? 1 : 4);
       for(size_t q = 0; q < order.size(); ++q)
       {
         const size_t at = size_t(ls->color_ptr[c]) + q, j = size_t(order[q]);

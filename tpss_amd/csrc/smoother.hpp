@@ -32,7 +32,8 @@ struct LocalSolvers
   DeviceArray<uint32_t> d_eo;      // nfac: 1 = eigenpairs in even-odd order
   // vtx and fac of every color in the launch order of the residual-free pass
   // (3D, K = 8, Cartesian): each color's patches dealt to the 8 XCDs in column
-  // tiles, z-slab by z-slab inside a tile (empty where that pass does not run)
+  // tiles (y bands for FP64 patch solves, 4 x 2 tiles for FP32), z-slab by
+  // z-slab inside a tile (empty where that pass does not run)
   DeviceArray<uint32_t> d_vtx_launch, d_fac_launch;
   std::vector<double> h_host;      // P * dim * 2 surrogate lengths
   std::vector<double> fit_err;     // P: relative error of the separable fit (-1: surrogate kept)
